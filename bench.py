@@ -1,0 +1,244 @@
+"""Benchmark of the MI355X SpMV path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+
+A step is one SpMV over the whole matrix with operands resident in HBM.
+Headline workload: FP64 SELL-C-sigma SpMV on the 4M-row Lomax band matrix
+(N = 4,000,000, row lengths Lomax(1.5) with mean 16 and cap 2000, columns
+uniform within +-N/8, values and x uniform [-1, 1)).  The value is algorithmic
+GB/s: B_alg = nnz*(8+4) + (N+1)*4 + N*8 (x) + N*8 (y) per SpMV, divided by
+the average kernel time measured with HIP events on the library's stream.
+
+Also reported: every (C, sigma) variant, CRS (lane-per-row and LDS row-block
+schedules), the 512^2 5-point Laplacian (warm and cold L2/MALL), the device
+copy bandwidth used as the measured stream peak, and the oracle's OpenMP CRS
+SpMV on the host cores (bounded sample) as the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from smaxkernels_amd import capi, matgen  # noqa: E402
+
+METRIC = ("SpMV GB/s & % HBM peak; BF16-refined vs FP64 CG speedup at 1e-10 "
+          "residual")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def b_alg(n_rows, n_cols, nnz, vb=8, ib=4, xb=8, yb=8) -> int:
+    return nnz * (vb + ib) + (n_rows + 1) * ib + n_cols * xb + n_rows * yb
+
+
+def time_launches(ctx, fn, steps, warmup) -> float:
+    """Average ms per launch over `steps` back-to-back launches."""
+    for _ in range(warmup):
+        fn()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(steps):
+        fn()
+    return ctx.timer_stop() / steps
+
+
+def stream_peak(ctx, steps=10) -> dict:
+    n = 1 << 29  # 4 GiB per array
+    a = ctx.zeros(n, np.float64)
+    b = ctx.zeros(n, np.float64)
+    ms = min(time_launches(ctx, lambda: ctx.stream_copy(a, b, n), 1, 2)
+             for _ in range(steps))
+    copy_gbs = 2 * n * 8 / ms / 1e6
+    c = ctx.zeros(n, np.float64)
+    ms3 = min(time_launches(ctx, lambda: ctx.stream_triad(a, b, c, 0.5, n), 1, 2)
+              for _ in range(steps))
+    triad_gbs = 3 * n * 8 / ms3 / 1e6
+    for d in (a, b, c):
+        d.free()
+    return {"copy_GBs": round(copy_gbs, 1), "triad_GBs": round(triad_gbs, 1),
+            "array_GiB": 4}
+
+
+def cpu_baseline(A, x, seconds) -> dict:
+    """The oracle's OpenMP CRS SpMV (reference loop order) on host cores."""
+    from oracle import oracle
+    y = np.empty(A.n_rows)
+    oracle.crs_spmv(A.row_ptr, A.col, A.val, x)  # warm-up
+    n_iter, runtime = 1, 0.0
+    while True:
+        t0 = time.perf_counter()
+        for _ in range(n_iter):
+            y = oracle.crs_spmv(A.row_ptr, A.col, A.val, x)
+        runtime = time.perf_counter() - t0
+        if runtime >= seconds or n_iter >= 1 << 20:
+            break
+        n_iter *= 2
+    t = runtime / n_iter
+    gbs = b_alg(A.n_rows, A.n_cols, A.nnz) / t / 1e9
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{n_iter} CRS SpMVs of the N=4M Lomax matrix "
+                      f"({runtime:.1f} s, {t * 1e3:.2f} ms each, "
+                      f"{2 * A.nnz / t / 1e9:.2f} GFLOP/s), OpenMP static rows"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--n", type=int, default=4_000_000)
+    ap.add_argument("--full-sweep", action="store_true",
+                    help="also time sigma in {1, 64} (17x padded, 13 GB each)")
+    ap.add_argument("--only", default="",
+                    help="comma list of sections to run: stream,sell,crs,c1,cpu")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(row-block sharding is described in DESIGN.md).", file=sys.stderr)
+        return 2
+    only = set(filter(None, args.only.split(",")))
+    want = (lambda s: not only or s in only)
+    extra: dict = {}
+    ctx = capi.Context(0)
+
+    peak = None
+    if want("stream"):
+        log("stream peak (4 GiB arrays)")
+        peak = stream_peak(ctx)
+        extra["stream"] = peak
+    peak_gbs = peak["copy_GBs"] if peak else None
+
+    log(f"generating Lomax band matrix N={args.n}")
+    A = matgen.lomax_band(args.n)
+    x = matgen.uniform_f64(matgen.SEED_X, A.n_cols)
+    balg = b_alg(A.n_rows, A.n_cols, A.nnz)
+    rp, col, val = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
+    dx = ctx.upload(x)
+    dy = ctx.zeros(A.n_rows + 64, np.float64)
+
+    best = None
+    sell = {}
+    if want("sell"):
+        sigmas = [4096, A.n_rows] + ([1, 64] if args.full_sweep else [])
+        for C_ in (64, 32):
+            for sigma in sigmas:
+                t0 = time.perf_counter()
+                s = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 8,
+                                     C_, sigma)
+                conv_s = time.perf_counter() - t0
+                yy = ctx.zeros(s.info.n_rows_padded, np.float64)
+                ms = time_launches(ctx, lambda: s.spmv(capi.F64, dx, yy),
+                                   args.steps, args.warmup)
+                gbs = balg / ms / 1e6
+                fmt = (s.info.n_elements * 12 + (s.info.n_chunks + 1) * 4 +
+                       s.info.n_chunks * 4 + A.n_cols * 8 +
+                       s.info.n_rows_padded * 8)
+                key = f"C{C_}_s{'N' if sigma == A.n_rows else sigma}"
+                sell[key] = {"ms": round(ms, 4), "GBs_alg": round(gbs, 1),
+                             "GFLOPs": round(2 * A.nnz / ms / 1e6, 1),
+                             "GBs_format": round(fmt / ms / 1e6, 1),
+                             "fill": round(s.info.n_elements / A.nnz, 4),
+                             "conversion_s": round(conv_s, 3)}
+                log(f"SELL {key}: {ms:.4f} ms {gbs:.1f} GB/s")
+                if best is None or ms < best[1]:
+                    best = (key, ms, C_, sigma)
+                yy.free()
+                s.destroy()
+        extra["sell_f64"] = sell
+
+    crs = {}
+    if want("crs"):
+        ms = time_launches(ctx, lambda: ctx.crs_spmv(
+            capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, val, dx, dy, None),
+            args.steps, args.warmup)
+        crs["lane_per_row"] = {"ms": round(ms, 4), "GBs_alg": round(balg / ms / 1e6, 1)}
+        plan = ctx.crs_plan(A.n_rows, A.nnz, rp)
+        ms = time_launches(ctx, lambda: ctx.crs_spmv(
+            capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, val, dx, dy, plan),
+            args.steps, args.warmup)
+        crs["row_blocks"] = {"ms": round(ms, 4), "GBs_alg": round(balg / ms / 1e6, 1)}
+        ctx.crs_plan_destroy(plan)
+        log(f"CRS {crs}")
+        extra["crs_f64"] = crs
+
+    if want("c1"):
+        P = matgen.poisson2d(512)
+        px = matgen.uniform_f64(matgen.SEED_X, P.n_cols)
+        pb = b_alg(P.n_rows, P.n_cols, P.nnz)
+        prp, pcol, pval, pdx = (ctx.upload(P.row_ptr), ctx.upload(P.col),
+                                ctx.upload(P.val), ctx.upload(px))
+        ps = ctx.scs_from_crs(P.n_rows, P.n_cols, P.nnz, prp, pcol, pval, 8, 64, 1)
+        pdy = ctx.zeros(ps.info.n_rows_padded, np.float64)
+        warm = time_launches(ctx, lambda: ps.spmv(capi.F64, pdx, pdy), 2000, 100)
+        scrub_n = 1 << 27  # 1 GiB read + 1 GiB written between launches
+        s1, s2 = ctx.zeros(scrub_n, np.float64), ctx.zeros(scrub_n, np.float64)
+        cold = []
+        for _ in range(20):
+            ctx.stream_copy(s1, s2, scrub_n)
+            ctx.timer_start()
+            ps.spmv(capi.F64, pdx, pdy)
+            cold.append(ctx.timer_stop())
+        cold_ms = float(np.median(cold))
+        extra["c1_poisson512_f64_sell64_1"] = {
+            "B_alg": pb, "warm_ms": round(warm, 5),
+            "warm_GBs": round(pb / warm / 1e6, 1),
+            "cold_ms": round(cold_ms, 5), "cold_GBs": round(pb / cold_ms / 1e6, 1),
+            "cold_method": "1 GiB copy between launches, median of 20 event-timed"}
+        for d in (s1, s2, pdy, prp, pcol, pval, pdx):
+            d.free()
+        ps.destroy()
+        log(f"C1 {extra['c1_poisson512_f64_sell64_1']}")
+
+    base = None
+    if want("cpu"):
+        log(f"CPU baseline ({args.cpu_seconds} s sample)")
+        base = cpu_baseline(A, x, args.cpu_seconds)
+
+    if best is None:
+        key, ms = "crs_row_blocks", crs.get("row_blocks", {}).get("ms", float("nan"))
+        C_, sigma = None, None
+    else:
+        key, ms, C_, sigma = best
+    value = balg / ms / 1e6
+    roof = {"bound": "hbm", "achieved": round(value, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(value / HBM_PEAK_GBS, 4),
+            "traffic": None}
+    if peak_gbs:
+        roof["frac_of_measured_stream"] = round(value / peak_gbs, 4)
+        roof["measured_stream_GBs"] = peak_gbs
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "GB/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"lomax_band N={A.n_rows} nnz={A.nnz} FP64 "
+                               f"SELL-C-sigma {key}",
+                   "n_rows": A.n_rows, "nnz": A.nnz, "C": C_, "sigma": sigma,
+                   "B_alg_bytes": balg},
+        "roofline": roof,
+        "cpu_baseline": base,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

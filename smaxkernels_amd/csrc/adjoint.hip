@@ -1,0 +1,298 @@
+// Differentiation support: the SELL-C-sigma layout of A^T and the SDDMM
+// value gradient on A's SELL layout.  Neither has a reference counterpart.
+//
+// A^T: the CRS positions j = 0..nnz-1 are already in row-major order, so a
+// *stable* sort of (col[j], j) by column yields A^T's CRS with every column's
+// entries in ascending original row order.  The sort is an LSD radix sort
+// with 4-bit digits whose scatter ranks come from prefix sums only (no
+// atomics decide an order), so the layout is deterministic.  The resulting
+// CRS of A^T goes through the same GPU SELL conversion as A, with src_index
+// pointing back at A's CRS nonzeros.
+//
+// SDDMM: dvals[src_index[e]] = sum_v G[r*k+v] * X[col[e]*k+v] for every stored
+// element e of SELL row r; a group of up to 64 lanes takes one element, each
+// lane a strided part of v (fma chain), then a fixed xor-butterfly sums the
+// group: the reduction tree depends only on k.
+#include <utility>
+
+#include "common.hpp"
+#include "convert_internal.hpp"
+#include "scan.hpp"
+
+namespace smaxhip {
+
+constexpr int kRThreads = 256;
+constexpr int kRItems = 16;
+constexpr int kRTile = kRThreads * kRItems;
+constexpr int kRBuckets = 16;
+
+__global__ __launch_bounds__(kRThreads) void radix_hist(
+    int64_t n, int shift, const uint32_t *__restrict__ keys,
+    int32_t *__restrict__ hist, int64_t n_tiles) {
+    __shared__ int32_t c[kRBuckets];
+    if (threadIdx.x < kRBuckets) c[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kRTile +
+                         static_cast<int64_t>(threadIdx.x) * kRItems;
+    for (int k = 0; k < kRItems; ++k) {
+        const int64_t i = base + k;
+        if (i < n) atomicAdd(&c[(keys[i] >> shift) & (kRBuckets - 1)], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kRBuckets)
+        hist[threadIdx.x * n_tiles + blockIdx.x] = c[threadIdx.x];
+}
+
+// incl: inclusive scan of hist (digit-major), so (digit d, tile t) starts at
+// incl[d*n_tiles+t] - hist[d*n_tiles+t].
+__global__ __launch_bounds__(kRThreads) void radix_scatter(
+    int64_t n, int shift, const uint32_t *__restrict__ keys,
+    const int32_t *__restrict__ vals, const int32_t *__restrict__ incl,
+    const int32_t *__restrict__ hist, int64_t n_tiles,
+    uint32_t *__restrict__ okeys, int32_t *__restrict__ ovals) {
+    __shared__ int32_t cnt[kRBuckets * kRThreads];
+    const int tid = threadIdx.x;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kRTile +
+                         static_cast<int64_t>(tid) * kRItems;
+    uint32_t kk[kRItems];
+    int32_t vv[kRItems];
+    int32_t mine[kRBuckets];
+    for (int d = 0; d < kRBuckets; ++d) mine[d] = 0;
+    for (int k = 0; k < kRItems; ++k) {
+        const int64_t i = base + k;
+        if (i < n) {
+            kk[k] = keys[i];
+            vv[k] = vals[i];
+            mine[(kk[k] >> shift) & (kRBuckets - 1)]++;
+        }
+    }
+    for (int d = 0; d < kRBuckets; ++d) cnt[d * kRThreads + tid] = mine[d];
+    __syncthreads();
+    // flat exclusive scan of cnt (digit-major): thread t owns [16t, 16t+16)
+    int32_t loc[kRItems];
+    int32_t s = 0;
+    for (int k = 0; k < kRItems; ++k) {
+        loc[k] = s;
+        s += cnt[tid * kRItems + k];
+    }
+    int32_t total;
+    const int32_t before = block_excl_scan(s, &total);
+    __syncthreads();
+    for (int k = 0; k < kRItems; ++k) cnt[tid * kRItems + k] = before + loc[k];
+    __syncthreads();
+    int32_t run[kRBuckets];
+    for (int d = 0; d < kRBuckets; ++d)
+        run[d] = incl[d * n_tiles + blockIdx.x] - hist[d * n_tiles + blockIdx.x] +
+                 cnt[d * kRThreads + tid] - cnt[d * kRThreads];
+    for (int k = 0; k < kRItems; ++k) {
+        const int64_t i = base + k;
+        if (i < n) {
+            const int d = (kk[k] >> shift) & (kRBuckets - 1);
+            const int32_t p = run[d]++;
+            okeys[p] = kk[k];
+            ovals[p] = vv[k];
+        }
+    }
+}
+
+__global__ void iota_keys(int64_t n, const int32_t *__restrict__ col,
+                          uint32_t *__restrict__ keys, int32_t *__restrict__ vals) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = static_cast<uint32_t>(col[i]);
+    vals[i] = static_cast<int32_t>(i);
+}
+
+// t_row_ptr[c] = first sorted position with key >= c, for c in [0, n_cols].
+__global__ void transpose_row_ptr(int64_t n_cols, int64_t nnz,
+                                  const uint32_t *__restrict__ keys,
+                                  int32_t *__restrict__ t_row_ptr) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c > n_cols) return;
+    int64_t a = 0, b = nnz;
+    while (a < b) {
+        const int64_t m = (a + b) >> 1;
+        if (keys[m] < static_cast<uint32_t>(c)) a = m + 1; else b = m;
+    }
+    t_row_ptr[c] = static_cast<int32_t>(a);
+}
+
+__global__ void transpose_cols(int64_t nnz, int64_t n_rows,
+                               const int32_t *__restrict__ row_ptr,
+                               const int32_t *__restrict__ src,
+                               int32_t *__restrict__ t_col) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    const int64_t j = src[p];
+    int64_t lo = 0, hi = n_rows;
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (row_ptr[mid] <= j) lo = mid; else hi = mid;
+    }
+    t_col[p] = static_cast<int32_t>(lo);
+}
+
+// ------------------------------------------------------------------ SDDMM
+template <int VT, int G>
+__global__ __launch_bounds__(256) void scs_sddmm_kernel(
+    int64_t n_elements, int64_t C, int64_t n_chunks, int64_t k,
+    const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ slot_row, const int32_t *__restrict__ src,
+    const typename vt<VT>::vec_t *__restrict__ Gm,
+    const typename vt<VT>::vec_t *__restrict__ X,
+    typename vt<VT>::vec_t *__restrict__ dvals) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t e = t / G;
+    const int lane = static_cast<int>(t % G);
+    const bool valid = e < n_elements;
+    const int32_t s = valid ? src[e] : -1;
+    A acc = 0;
+    if (s >= 0) {
+        int64_t a = 0, b = n_chunks;  // chunk: chunk_ptr[a] <= e < chunk_ptr[a+1]
+        while (b - a > 1) {
+            const int64_t m = (a + b) >> 1;
+            if (chunk_ptr[m] <= e) a = m; else b = m;
+        }
+        // skip chunks of zero length that share the same start
+        while (chunk_ptr[a + 1] <= e) ++a;
+        const int64_t slot = a * C + (e - chunk_ptr[a]) % C;
+        const int64_t r = slot_row[slot];
+        const int64_t c = col[e];
+        for (int64_t v = lane; v < k; v += G)
+            acc = T::fma_(Gm[r * k + v], X[c * k + v], acc);
+    }
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
+    if (s >= 0 && lane == 0) dvals[s] = acc;
+}
+
+}  // namespace smaxhip
+
+using namespace smaxhip;
+
+extern "C" {
+
+int smax_hip_scs_transpose_from_crs(smax_hip_ctx ctx, int64_t n_rows,
+                                    int64_t n_cols, int64_t nnz,
+                                    const int32_t *row_ptr, const int32_t *col,
+                                    const void *val, int64_t val_bytes,
+                                    int64_t C, int64_t sigma,
+                                    smax_hip_scs *out) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, out, "smax_hip_scs_transpose_from_crs: null output");
+    *out = nullptr;
+    SMAX_HIP_REQUIRE(ctx, n_rows > 0 && n_cols > 0 && nnz >= 0 &&
+                              nnz <= INT32_MAX && row_ptr &&
+                              (nnz == 0 || (col && val)),
+                     "smax_hip_scs_transpose_from_crs: bad input");
+    const int64_t n_tiles = ceil_div(nnz > 0 ? nnz : 1, kRTile);
+    uint32_t *k0 = nullptr, *k1 = nullptr;
+    int32_t *v0 = nullptr, *v1 = nullptr, *hist = nullptr, *tmp = nullptr;
+    int32_t *incl = nullptr;
+    int32_t *t_row_ptr = nullptr, *t_col = nullptr;
+    const int64_t m = nnz > 0 ? nnz : 1;
+    int rc = dev_alloc(ctx, reinterpret_cast<void **>(&k0), 4 * m, "transpose");
+    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&k1), 4 * m, "transpose");
+    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&v0), 4 * m, "transpose");
+    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&v1), 4 * m, "transpose");
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&hist),
+                       4 * kRBuckets * n_tiles, "transpose");
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&incl),
+                       4 * kRBuckets * n_tiles, "transpose");
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&tmp),
+                       4 * scan_tmp_elems(kRBuckets * n_tiles), "transpose");
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&t_row_ptr),
+                       4 * (n_cols + 1), "transpose");
+    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&t_col), 4 * m, "transpose");
+    if (!rc && nnz > 0) {
+        hipLaunchKernelGGL(iota_keys, grid_for(nnz, 256), dim3(256), 0,
+                           ctx->stream, nnz, col, k0, v0);
+        rc = check_launch(ctx, "iota_keys");
+        int bits = 1;
+        while (bits < 32 && (1ll << bits) < n_cols) ++bits;
+        for (int shift = 0; !rc && shift < bits; shift += 4) {
+            hipLaunchKernelGGL(radix_hist, dim3(static_cast<unsigned>(n_tiles)),
+                               dim3(kRThreads), 0, ctx->stream, nnz, shift, k0,
+                               hist, n_tiles);
+            rc = check_launch(ctx, "radix_hist");
+            if (!rc)
+                rc = scan_inclusive<int32_t, false>(ctx, hist, incl,
+                                                    kRBuckets * n_tiles, tmp);
+            if (!rc) {
+                hipLaunchKernelGGL(radix_scatter,
+                                   dim3(static_cast<unsigned>(n_tiles)),
+                                   dim3(kRThreads), 0, ctx->stream, nnz, shift,
+                                   k0, v0, incl, hist, n_tiles, k1, v1);
+                rc = check_launch(ctx, "radix_scatter");
+            }
+            std::swap(k0, k1);
+            std::swap(v0, v1);
+        }
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(transpose_row_ptr, grid_for(n_cols + 1, 256), dim3(256),
+                           0, ctx->stream, n_cols, nnz, k0, t_row_ptr);
+        rc = check_launch(ctx, "transpose_row_ptr");
+    }
+    if (!rc && nnz > 0) {
+        hipLaunchKernelGGL(transpose_cols, grid_for(nnz, 256), dim3(256), 0,
+                           ctx->stream, nnz, n_rows, row_ptr, v0, t_col);
+        rc = check_launch(ctx, "transpose_cols");
+    }
+    if (!rc)
+        rc = build_scs(ctx, n_cols, n_rows, nnz, t_row_ptr, t_col, val,
+                       val_bytes, v0, C, sigma, out);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : {static_cast<void *>(k0), static_cast<void *>(k1),
+                    static_cast<void *>(v0), static_cast<void *>(v1),
+                    static_cast<void *>(hist), static_cast<void *>(incl),
+                    static_cast<void *>(tmp),
+                    static_cast<void *>(t_row_ptr), static_cast<void *>(t_col)})
+        dev_free(p);
+    return rc;
+}
+
+int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
+                       int64_t k, const void *G, const void *X, void *dvals) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    if (!valid_value_type(value_type))
+        return fail(ctx, SMAX_HIP_ERROR_UNSUPPORTED,
+                    "smax_hip_scs_sddmm: value type %d", value_type);
+    SMAX_HIP_REQUIRE(ctx, A && k > 0 && G && X && dvals,
+                     "smax_hip_scs_sddmm: bad arguments");
+    const smax_hip_scs_info &in = A->info;
+    if (in.n_elements == 0) return SMAX_HIP_SUCCESS;
+    int g = 1;
+    while (g < k && g < 64) g <<= 1;
+    const int64_t threads = in.n_elements * g;
+    SMAX_HIP_DISPATCH_VT(value_type, {
+        using T = vt<VT>;
+        auto Gp = static_cast<const typename T::vec_t *>(G);
+        auto Xp = static_cast<const typename T::vec_t *>(X);
+        auto Dp = static_cast<typename T::vec_t *>(dvals);
+        const dim3 grid = grid_for(threads, 256);
+#define SDDMM_CASE(GG)                                                         \
+    case GG:                                                                   \
+        hipLaunchKernelGGL((scs_sddmm_kernel<VT, GG>), grid, dim3(256), 0,     \
+                           ctx->stream, in.n_elements, in.C, in.n_chunks, k,   \
+                           in.chunk_ptr, in.col, in.slot_row, in.src_index,    \
+                           Gp, Xp, Dp);                                        \
+        break;
+        switch (g) {
+            SDDMM_CASE(1) SDDMM_CASE(2) SDDMM_CASE(4) SDDMM_CASE(8)
+            SDDMM_CASE(16) SDDMM_CASE(32) SDDMM_CASE(64)
+        }
+#undef SDDMM_CASE
+        SMAX_HIP_LAUNCHED(ctx, "scs_sddmm_kernel");
+        return SMAX_HIP_SUCCESS;
+    });
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// BCRS SpMV / SpMM and CRS -> BCRS conversion for gfx950.
+//
+// Block row r, block j (CRS order over blocks), block column jx:
+//   y[r*h_pad + ix] = sum_j sum_jx val[j*h_pad*w_pad + off(ix, jx)] *
+//                     x[col[j]*w_pad + jx]
+// with off = jx*h_pad + ix (column-major blocks) or ix*w_pad + jx
+// (row-major), accumulated per output in (j, jx) order exactly as
+// naive_bcrs_spmv does (spmv_cpu_bcrs_impl.hpp:45-74): bit-identical.
+//
+// SpMV stages the blocks of a range of block rows through LDS: the values
+// of 256/b_h block rows are contiguous in memory, so the workgroup streams
+// them (and the x entries their columns select) with coalesced loads, then
+// lane (r, ix) runs its fma chain out of LDS.
+#include "common.hpp"
+#include "scan.hpp"
+
+struct smax_hip_bcrs_s {
+    smax_hip_bcrs_info info;
+};
+
+namespace smaxhip {
+
+constexpr int kBThreads = 256;
+constexpr int kBTileVals = 2048;  // block values staged per LDS tile
+
+template <int VT>
+__global__ __launch_bounds__(kBThreads) void bcrs_spmv_lds(
+    int64_t n_block_rows, int b_h, int b_w, int h_pad, int w_pad, int col_major,
+    const int32_t *__restrict__ col, const int32_t *__restrict__ row_ptr,
+    const typename vt<VT>::val_t *__restrict__ val,
+    const typename vt<VT>::vec_t *__restrict__ x,
+    typename vt<VT>::vec_t *__restrict__ y) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    __shared__ A sv[kBTileVals];
+    __shared__ A sx[kBTileVals];
+    const int rows_per_wg = kBThreads / b_h;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_wg;
+    const int64_t r1 = min<int64_t>(r0 + rows_per_wg, n_block_rows);
+    const int tid = threadIdx.x;
+    const int64_t my_r = r0 + tid / b_h;
+    const int ix = tid % b_h;
+    const bool has = tid < rows_per_wg * b_h && my_r < r1;
+    const int64_t bs = static_cast<int64_t>(h_pad) * w_pad;
+    const int64_t s = row_ptr[r0], e = row_ptr[r1];
+    const int64_t js = has ? row_ptr[my_r] : 0, je = has ? row_ptr[my_r + 1] : 0;
+    // blocks per tile: values and gathered x entries both fit
+    const int64_t tb = min<int64_t>(kBTileVals / bs, kBTileVals / w_pad);
+    A acc = 0;
+    for (int64_t t0 = s; t0 < e; t0 += tb) {
+        const int64_t nb = min<int64_t>(tb, e - t0);
+        const int64_t nv = nb * bs;
+        const typename T::val_t *vsrc = val + t0 * bs;
+        for (int64_t k = tid; k < nv; k += kBThreads) sv[k] = T::widen(ld_stream(vsrc + k));
+        const int64_t nx = nb * w_pad;
+        for (int64_t k = tid; k < nx; k += kBThreads) {
+            const int64_t b = k / w_pad, jx = k - b * w_pad;
+            sx[k] = jx < b_w ? x[static_cast<int64_t>(col[t0 + b]) * w_pad + jx] : A(0);
+        }
+        __syncthreads();
+        if (has) {
+            const int64_t lo = max(js, t0), hi = min(je, t0 + nb);
+            for (int64_t j = lo; j < hi; ++j) {
+                const int64_t b = j - t0;
+                for (int jx = 0; jx < b_w; ++jx) {
+                    const int64_t off = col_major ? jx * h_pad + ix : ix * w_pad + jx;
+                    acc = T::fma_(sv[b * bs + off], sx[b * w_pad + jx], acc);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (has) y[my_r * h_pad + ix] = acc;
+}
+
+// Thread (r, ix, v), v fastest: the k lanes of one output row share each
+// block value (broadcast) and read one contiguous row of X per (j, jx).
+template <int VT>
+__global__ __launch_bounds__(kBThreads) void bcrs_spmm_kernel(
+    int64_t n_block_rows, int b_h, int b_w, int h_pad, int w_pad, int col_major,
+    int64_t k, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ row_ptr,
+    const typename vt<VT>::val_t *__restrict__ val,
+    const typename vt<VT>::vec_t *__restrict__ X,
+    typename vt<VT>::vec_t *__restrict__ Y) {
+    using T = vt<VT>;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
+    const int64_t v = t % k, rix = t / k;
+    const int64_t r = rix / b_h;
+    const int ix = static_cast<int>(rix - r * b_h);
+    if (r >= n_block_rows) return;
+    const int64_t bs = static_cast<int64_t>(h_pad) * w_pad;
+    typename T::vec_t acc = 0;
+    for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; ++j) {
+        const typename T::val_t *blk = val + j * bs;
+        const typename T::vec_t *xr =
+            X + static_cast<int64_t>(col[j]) * w_pad * k + v;
+        for (int jx = 0; jx < b_w; ++jx) {
+            const int64_t off = col_major ? jx * h_pad + ix : ix * w_pad + jx;
+            acc = T::fma_(T::widen(blk[off]), xr[jx * k], acc);
+        }
+    }
+    Y[(r * h_pad + ix) * k + v] = acc;
+}
+
+// ------------------------------------------------------------ conversion
+// Sorted unique block columns of block row br: a merge of its b_h rows, whose
+// block-column sequences must be non-decreasing (the reference's fill loop,
+// convert_formats.hpp:222-234, throws otherwise).  Writes them to out (if
+// non-null) and returns their count, or -1 on an out-of-order row.
+__device__ int64_t merge_block_row(const int32_t *__restrict__ row_ptr,
+                                   const int32_t *__restrict__ col, int64_t br,
+                                   int b_h, int b_w, int32_t *out) {
+    constexpr int kMaxBh = 64;
+    int64_t pos[kMaxBh], end[kMaxBh];
+    for (int l = 0; l < b_h; ++l) {
+        pos[l] = row_ptr[br * b_h + l];
+        end[l] = row_ptr[br * b_h + l + 1];
+        for (int64_t j = pos[l] + 1; j < end[l]; ++j)
+            if (col[j] / b_w < col[j - 1] / b_w) return -1;
+    }
+    int64_t u = 0;
+    int32_t last = -1;
+    for (;;) {
+        int32_t m = INT32_MAX;
+        for (int l = 0; l < b_h; ++l)
+            if (pos[l] < end[l]) m = min(m, col[pos[l]] / b_w);
+        if (m == INT32_MAX) break;
+        for (int l = 0; l < b_h; ++l)
+            while (pos[l] < end[l] && col[pos[l]] / b_w == m) ++pos[l];
+        if (m != last) {
+            if (out) out[u] = m;
+            ++u;
+            last = m;
+        }
+    }
+    return u;
+}
+
+__global__ void bcrs_count(int64_t nbr, int b_h, int b_w,
+                           const int32_t *__restrict__ row_ptr,
+                           const int32_t *__restrict__ col,
+                           int64_t *__restrict__ cnt, int64_t *__restrict__ bad) {
+    const int64_t br = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (br >= nbr) return;
+    const int64_t u = merge_block_row(row_ptr, col, br, b_h, b_w, nullptr);
+    if (u < 0) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(bad), 1ull);
+        cnt[br] = 0;
+    } else {
+        cnt[br] = u;
+    }
+}
+
+__global__ void bcrs_fill_cols(int64_t nbr, int b_h, int b_w,
+                               const int32_t *__restrict__ row_ptr,
+                               const int32_t *__restrict__ col,
+                               const int64_t *__restrict__ incl,
+                               int32_t *__restrict__ brow_ptr,
+                               int32_t *__restrict__ bcol) {
+    const int64_t br = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (br >= nbr) return;
+    const int64_t start = br == 0 ? 0 : incl[br - 1];
+    brow_ptr[br + 1] = static_cast<int32_t>(incl[br]);
+    if (br == 0) brow_ptr[0] = 0;
+    merge_block_row(row_ptr, col, br, b_h, b_w, bcol + start);
+}
+
+template <typename V>
+__global__ void bcrs_fill_vals(int64_t nnz, int64_t n_rows, int b_h, int b_w,
+                               int h_pad, int w_pad, int col_major,
+                               const int32_t *__restrict__ row_ptr,
+                               const int32_t *__restrict__ col,
+                               const V *__restrict__ val,
+                               const int32_t *__restrict__ brow_ptr,
+                               const int32_t *__restrict__ bcol,
+                               V *__restrict__ bval) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= nnz) return;
+    int64_t lo = 0, hi = n_rows;  // row of j: row_ptr[lo] <= j < row_ptr[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (row_ptr[mid] <= j) lo = mid; else hi = mid;
+    }
+    const int64_t r = lo, br = r / b_h, lr = r - br * b_h;
+    const int32_t c = col[j], bc = c / b_w;
+    int64_t a = brow_ptr[br], b = brow_ptr[br + 1];  // lower_bound of bc
+    while (a < b) {
+        const int64_t m = (a + b) >> 1;
+        if (bcol[m] < bc) a = m + 1; else b = m;
+    }
+    const int64_t inc = col_major ? static_cast<int64_t>(h_pad) * (c % b_w) + lr
+                                  : static_cast<int64_t>(w_pad) * lr + (c % b_w);
+    bval[a * h_pad * w_pad + inc] = val[j];
+}
+
+static int check_bcrs(smax_hip_ctx ctx, const char *fn, int value_type,
+                      int64_t nbr, int64_t b_h, int64_t b_w, int64_t h_pad,
+                      int64_t w_pad) {
+    if (!valid_value_type(value_type))
+        return fail(ctx, SMAX_HIP_ERROR_UNSUPPORTED, "%s: value type %d", fn,
+                    value_type);
+    SMAX_HIP_REQUIRE(ctx, nbr > 0 && b_h > 0 && b_w > 0 && h_pad >= b_h &&
+                              w_pad >= b_w,
+                     "%s: bad block shape (%lld rows, %lldx%lld pad %lldx%lld)",
+                     fn, (long long)nbr, (long long)b_h, (long long)b_w,
+                     (long long)h_pad, (long long)w_pad);
+    SMAX_HIP_REQUIRE(ctx, b_h <= kBThreads && h_pad * w_pad <= kBTileVals,
+                     "%s: blocks larger than %d values are not supported", fn,
+                     kBTileVals);
+    return SMAX_HIP_SUCCESS;
+}
+
+}  // namespace smaxhip
+
+using namespace smaxhip;
+
+extern "C" {
+
+int smax_hip_bcrs_spmv(smax_hip_ctx ctx, int value_type,
+                       int64_t n_block_rows, int64_t b_h, int64_t b_w,
+                       int64_t h_pad, int64_t w_pad, const int32_t *col,
+                       const int32_t *row_ptr, const void *val,
+                       int block_col_major, const void *x, void *y) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    int rc = check_bcrs(ctx, "smax_hip_bcrs_spmv", value_type, n_block_rows,
+                        b_h, b_w, h_pad, w_pad);
+    if (rc) return rc;
+    SMAX_HIP_REQUIRE(ctx, row_ptr && x && y, "smax_hip_bcrs_spmv: null pointer");
+    const int64_t rows_per_wg = kBThreads / b_h;
+    SMAX_HIP_DISPATCH_VT(value_type, {
+        using T = vt<VT>;
+        hipLaunchKernelGGL((bcrs_spmv_lds<VT>), grid_for(n_block_rows, rows_per_wg),
+                           dim3(kBThreads), 0, ctx->stream, n_block_rows,
+                           static_cast<int>(b_h), static_cast<int>(b_w),
+                           static_cast<int>(h_pad), static_cast<int>(w_pad),
+                           block_col_major ? 1 : 0, col, row_ptr,
+                           static_cast<const typename T::val_t *>(val),
+                           static_cast<const typename T::vec_t *>(x),
+                           static_cast<typename T::vec_t *>(y));
+        SMAX_HIP_LAUNCHED(ctx, "bcrs_spmv_lds");
+        return SMAX_HIP_SUCCESS;
+    });
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+int smax_hip_bcrs_spmm(smax_hip_ctx ctx, int value_type,
+                       int64_t n_block_rows, int64_t b_h, int64_t b_w,
+                       int64_t h_pad, int64_t w_pad, const int32_t *col,
+                       const int32_t *row_ptr, const void *val,
+                       int block_col_major, int64_t k, const void *X, void *Y) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    int rc = check_bcrs(ctx, "smax_hip_bcrs_spmm", value_type, n_block_rows,
+                        b_h, b_w, h_pad, w_pad);
+    if (rc) return rc;
+    SMAX_HIP_REQUIRE(ctx, k > 0 && row_ptr && X && Y,
+                     "smax_hip_bcrs_spmm: bad k or null pointer");
+    const int64_t threads = n_block_rows * b_h * k;
+    SMAX_HIP_DISPATCH_VT(value_type, {
+        using T = vt<VT>;
+        hipLaunchKernelGGL((bcrs_spmm_kernel<VT>), grid_for(threads, kBThreads),
+                           dim3(kBThreads), 0, ctx->stream, n_block_rows,
+                           static_cast<int>(b_h), static_cast<int>(b_w),
+                           static_cast<int>(h_pad), static_cast<int>(w_pad),
+                           block_col_major ? 1 : 0, k, col, row_ptr,
+                           static_cast<const typename T::val_t *>(val),
+                           static_cast<const typename T::vec_t *>(X),
+                           static_cast<typename T::vec_t *>(Y));
+        SMAX_HIP_LAUNCHED(ctx, "bcrs_spmm_kernel");
+        return SMAX_HIP_SUCCESS;
+    });
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
+                           int64_t nnz, const int32_t *row_ptr,
+                           const int32_t *col, const void *val,
+                           int64_t val_bytes, int64_t b_h, int64_t b_w,
+                           int64_t h_pad, int64_t w_pad, int block_col_major,
+                           smax_hip_bcrs *out) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, out, "smax_hip_bcrs_from_crs: null output");
+    *out = nullptr;
+    SMAX_HIP_REQUIRE(ctx, n_rows > 0 && n_cols > 0 && nnz >= 0 && row_ptr &&
+                              (nnz == 0 || (col && val)),
+                     "smax_hip_bcrs_from_crs: bad input");
+    SMAX_HIP_REQUIRE(ctx, val_bytes == 2 || val_bytes == 4 || val_bytes == 8,
+                     "smax_hip_bcrs_from_crs: val_bytes must be 2, 4 or 8");
+    // convert_formats.hpp:142-157
+    SMAX_HIP_REQUIRE(ctx, b_h > 0 && b_w > 0 && n_rows % b_h == 0,
+                     "Target block height does not divide number of input rows");
+    SMAX_HIP_REQUIRE(ctx, n_cols % b_w == 0,
+                     "Target block width does not divide number of input columns");
+    SMAX_HIP_REQUIRE(ctx, h_pad >= b_h,
+                     "Height padding has to be at least target block height");
+    SMAX_HIP_REQUIRE(ctx, w_pad >= b_w,
+                     "Width padding has to be at least target block width");
+    SMAX_HIP_REQUIRE(ctx, b_h <= 64, "smax_hip_bcrs_from_crs: b_h > 64");
+    const int64_t nbr = n_rows / b_h;
+    int64_t *cnt = nullptr, *incl = nullptr, *tmp = nullptr, *bad = nullptr;
+    smax_hip_bcrs_info info{};
+    int rc = dev_alloc(ctx, reinterpret_cast<void **>(&cnt), 8 * nbr, "bcrs");
+    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&incl), 8 * nbr, "bcrs");
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&tmp),
+                       8 * scan_tmp_elems(nbr), "bcrs");
+    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&bad), 8, "bcrs");
+    if (!rc) rc = smax_hip_zero(ctx, bad, 8);
+    int64_t n_blocks = 0, nbad = 0;
+    if (!rc) {
+        hipLaunchKernelGGL(bcrs_count, grid_for(nbr, 128), dim3(128), 0,
+                           ctx->stream, nbr, static_cast<int>(b_h),
+                           static_cast<int>(b_w), row_ptr, col, cnt, bad);
+        rc = check_launch(ctx, "bcrs_count");
+    }
+    if (!rc) rc = scan_inclusive<int64_t, false>(ctx, cnt, incl, nbr, tmp);
+    if (!rc) rc = read_i64(ctx, bad, &nbad);
+    if (!rc) rc = read_i64(ctx, incl + nbr - 1, &n_blocks);
+    if (!rc && nbad)
+        rc = fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                  "Blocked Column array does not fit with crs columns "
+                  "(%lld block rows with descending columns)", (long long)nbad);
+    if (!rc && n_blocks * h_pad * w_pad > INT32_MAX * 4LL)
+        rc = fail(ctx, SMAX_HIP_ERROR_OVERFLOW, "bcrs: too many blocks");
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&info.row_ptr),
+                       4 * (nbr + 1), "bcrs row_ptr");
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&info.col), 4 * n_blocks,
+                       "bcrs col");
+    if (!rc)
+        rc = dev_alloc(ctx, &info.val, val_bytes * n_blocks * h_pad * w_pad,
+                       "bcrs val");
+    if (!rc) rc = smax_hip_zero(ctx, info.val, val_bytes * n_blocks * h_pad * w_pad);
+    if (!rc) {
+        hipLaunchKernelGGL(bcrs_fill_cols, grid_for(nbr, 128), dim3(128), 0,
+                           ctx->stream, nbr, static_cast<int>(b_h),
+                           static_cast<int>(b_w), row_ptr, col, incl,
+                           info.row_ptr, info.col);
+        rc = check_launch(ctx, "bcrs_fill_cols");
+    }
+    if (!rc && nnz > 0) {
+        const dim3 g = grid_for(nnz, 256);
+        const int bh = static_cast<int>(b_h), bw = static_cast<int>(b_w);
+        const int hp = static_cast<int>(h_pad), wp = static_cast<int>(w_pad);
+        const int cm = block_col_major ? 1 : 0;
+        if (val_bytes == 8)
+            hipLaunchKernelGGL(bcrs_fill_vals<uint64_t>, g, dim3(256), 0,
+                               ctx->stream, nnz, n_rows, bh, bw, hp, wp, cm,
+                               row_ptr, col, static_cast<const uint64_t *>(val),
+                               info.row_ptr, info.col,
+                               static_cast<uint64_t *>(info.val));
+        else if (val_bytes == 4)
+            hipLaunchKernelGGL(bcrs_fill_vals<uint32_t>, g, dim3(256), 0,
+                               ctx->stream, nnz, n_rows, bh, bw, hp, wp, cm,
+                               row_ptr, col, static_cast<const uint32_t *>(val),
+                               info.row_ptr, info.col,
+                               static_cast<uint32_t *>(info.val));
+        else
+            hipLaunchKernelGGL(bcrs_fill_vals<uint16_t>, g, dim3(256), 0,
+                               ctx->stream, nnz, n_rows, bh, bw, hp, wp, cm,
+                               row_ptr, col, static_cast<const uint16_t *>(val),
+                               info.row_ptr, info.col,
+                               static_cast<uint16_t *>(info.val));
+        rc = check_launch(ctx, "bcrs_fill_vals");
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_free(cnt);
+    dev_free(incl);
+    dev_free(tmp);
+    dev_free(bad);
+    if (rc) {
+        dev_free(info.row_ptr);
+        dev_free(info.col);
+        dev_free(info.val);
+        return rc;
+    }
+    info.n_block_rows = nbr;
+    info.n_block_cols = n_cols / b_w;
+    info.n_blocks = n_blocks;
+    info.b_h = b_h;
+    info.b_w = b_w;
+    info.h_pad = h_pad;
+    info.w_pad = w_pad;
+    info.val_bytes = val_bytes;
+    info.block_col_major = block_col_major ? 1 : 0;
+    smax_hip_bcrs m = new smax_hip_bcrs_s();
+    m->info = info;
+    *out = m;
+    return SMAX_HIP_SUCCESS;
+}
+
+int smax_hip_bcrs_get_info(smax_hip_bcrs bcrs, smax_hip_bcrs_info *info) {
+    if (!bcrs || !info)
+        return fail(nullptr, SMAX_HIP_ERROR_INVALID_VALUE,
+                    "smax_hip_bcrs_get_info: null argument");
+    *info = bcrs->info;
+    return SMAX_HIP_SUCCESS;
+}
+
+int smax_hip_bcrs_destroy(smax_hip_ctx ctx, smax_hip_bcrs bcrs) {
+    if (!bcrs) return SMAX_HIP_SUCCESS;
+    if (ctx) {
+        (void)bind(ctx);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    dev_free(bcrs->info.row_ptr);
+    dev_free(bcrs->info.col);
+    dev_free(bcrs->info.val);
+    delete bcrs;
+    return SMAX_HIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// Internal helpers shared by the translation units of libsmax_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "smax_hip.h"
+
+struct smax_hip_ctx_s {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr;
+    hipEvent_t ev_stop = nullptr;
+    std::string last_error;
+    // Small pinned host word used to read back device counters.
+    int64_t *host_word = nullptr;
+};
+
+namespace smaxhip {
+
+// Records a failure on ctx (if any) and in the thread-local slot.
+int fail(smax_hip_ctx ctx, int status, const char *fmt, ...);
+
+// Maps a HIP error to a status and records it.
+int hip_fail(smax_hip_ctx ctx, hipError_t err, const char *what);
+
+// Makes ctx's device current for the calling thread.
+int bind(smax_hip_ctx ctx);
+
+// Checks the launch that was just enqueued.
+int check_launch(smax_hip_ctx ctx, const char *kernel);
+
+// Allocation helpers that record failures.
+int dev_alloc(smax_hip_ctx ctx, void **p, size_t bytes, const char *what);
+void dev_free(void *p);
+
+// Device counter read-back (int64 at a device address), synchronous.
+int read_i64(smax_hip_ctx ctx, const int64_t *dptr, int64_t *out);
+
+constexpr int kWave = 64;
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Grid for a 1-D launch with `threads` per block over n items.
+inline dim3 grid_for(int64_t n, int threads) {
+    int64_t g = ceil_div(n, threads);
+    return dim3(static_cast<unsigned>(g > 0 ? g : 1));
+}
+
+// ---------------------------------------------------------------- types
+// BF16 storage is the upper half of an IEEE binary32; widening is exact.
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+    return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+
+template <int VT> struct vt;
+template <> struct vt<SMAX_HIP_F64> {
+    using val_t = double;
+    using vec_t = double;
+    __device__ __forceinline__ static double widen(double v) { return v; }
+    __device__ __forceinline__ static double fma_(double a, double b,
+                                                  double c) {
+        return __builtin_fma(a, b, c);
+    }
+};
+template <> struct vt<SMAX_HIP_F32> {
+    using val_t = float;
+    using vec_t = float;
+    __device__ __forceinline__ static float widen(float v) { return v; }
+    __device__ __forceinline__ static float fma_(float a, float b, float c) {
+        return __builtin_fmaf(a, b, c);
+    }
+};
+template <> struct vt<SMAX_HIP_BF16> {
+    using val_t = uint16_t;
+    using vec_t = float;
+    __device__ __forceinline__ static float widen(uint16_t v) {
+        return bf16_to_f32(v);
+    }
+    __device__ __forceinline__ static float fma_(float a, float b, float c) {
+        return __builtin_fmaf(a, b, c);
+    }
+};
+
+inline bool valid_value_type(int v) {
+    return v == SMAX_HIP_F64 || v == SMAX_HIP_F32 || v == SMAX_HIP_BF16;
+}
+
+// Streamed-once loads bypass cache allocation.
+template <typename T> __device__ __forceinline__ T ld_stream(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
+}  // namespace smaxhip
+
+// Dispatch a templated launcher on the runtime value type.
+#define SMAX_HIP_DISPATCH_VT(vtype, ...)                                      \
+    switch (vtype) {                                                          \
+    case SMAX_HIP_F64: { constexpr int VT = SMAX_HIP_F64; __VA_ARGS__; break; } \
+    case SMAX_HIP_F32: { constexpr int VT = SMAX_HIP_F32; __VA_ARGS__; break; } \
+    case SMAX_HIP_BF16: { constexpr int VT = SMAX_HIP_BF16; __VA_ARGS__; break; } \
+    default: break;                                                           \
+    }
+
+#define SMAX_HIP_CHECK_CTX(ctx)                                               \
+    do {                                                                      \
+        if (!(ctx))                                                           \
+            return smaxhip::fail(nullptr, SMAX_HIP_ERROR_INVALID_VALUE,       \
+                                 "%s: null context", __func__);               \
+        int bind_rc_ = smaxhip::bind(ctx);                                    \
+        if (bind_rc_) return bind_rc_;                                        \
+    } while (0)
+
+#define SMAX_HIP_REQUIRE(ctx, cond, ...)                                      \
+    do {                                                                      \
+        if (!(cond))                                                          \
+            return smaxhip::fail((ctx), SMAX_HIP_ERROR_INVALID_VALUE,         \
+                                 __VA_ARGS__);                                \
+    } while (0)
+
+#define SMAX_HIP_CALL(ctx, expr)                                              \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) return smaxhip::hip_fail((ctx), e_, #expr);     \
+    } while (0)
+
+#define SMAX_HIP_LAUNCHED(ctx, name)                                          \
+    do {                                                                      \
+        int rc_ = smaxhip::check_launch((ctx), (name));                       \
+        if (rc_) return rc_;                                                  \
+    } while (0)

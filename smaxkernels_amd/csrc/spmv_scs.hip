@@ -1,0 +1,284 @@
+// SELL-C-sigma SpMV and SpMM for gfx950.
+//
+// Layout (src/utils/convert_formats.hpp:16-132): chunk c holds C slots; its
+// element j of slot i sits at chunk_ptr[c] + j*C + i, so for C = 64 one
+// wavefront reads a whole 64-slot column of a chunk with one coalesced
+// load per array.  Every slot is a sequential fma chain over j, exactly the
+// loop of the reference (spmv_cpu_scs_impl.hpp:27-41), so results are
+// bit-identical to it; padding elements (val 0, col 0) add an exact zero.
+//
+// The chains are unrolled: U column indices and values are loaded
+// (non-temporally, they are streamed once) before the U gathers of x and the
+// U dependent fmas, so each lane keeps 2U loads in flight.
+#include "common.hpp"
+#include "convert_internal.hpp"
+
+namespace smaxhip {
+
+constexpr int kScsThreads = 256;
+constexpr int kScsUnroll = 8;
+
+// Workgroup order remapped so each XCD (workgroups are dealt round-robin to
+// the 8 XCDs) walks one contiguous range of chunks: neighbouring chunks, which
+// gather overlapping parts of x, then share that XCD's L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks) {
+    constexpr int64_t kXcd = 8;
+    const int64_t xcd = bid % kXcd, local = bid / kXcd;
+    const int64_t q = nblocks / kXcd, r = nblocks % kXcd;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+template <int VT, int U>
+__device__ __forceinline__ typename vt<VT>::vec_t
+slot_chain(const int32_t *__restrict__ col,
+           const typename vt<VT>::val_t *__restrict__ val,
+           const typename vt<VT>::vec_t *__restrict__ x, int64_t base,
+           int64_t stride, int len) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    A acc = 0;
+    int j = 0;
+    for (; j + U <= len; j += U) {
+        int32_t c[U];
+        typename T::val_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = ld_stream(col + base + (j + u) * stride);
+            v[u] = ld_stream(val + base + (j + u) * stride);
+        }
+        A xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = T::fma_(T::widen(v[u]), xv[u], acc);
+    }
+    for (; j < len; ++j) {
+        const int32_t c = ld_stream(col + base + j * stride);
+        const auto v = ld_stream(val + base + j * stride);
+        acc = T::fma_(T::widen(v), x[c], acc);
+    }
+    return acc;
+}
+
+// C == 64: one wavefront per chunk, one lane per slot.
+template <int VT, bool UNPERM>
+__global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
+    int64_t n_chunks, int64_t n_rows, const int32_t *__restrict__ chunk_ptr,
+    const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ col,
+    const typename vt<VT>::val_t *__restrict__ val,
+    const int32_t *__restrict__ slot_row,
+    const typename vt<VT>::vec_t *__restrict__ x,
+    typename vt<VT>::vec_t *__restrict__ y) {
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t chunk = blk * (kScsThreads / kWave) + (threadIdx.x / kWave);
+    if (chunk >= n_chunks) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + lane;
+    const int len = chunk_len[chunk];
+    const auto acc = slot_chain<VT, kScsUnroll>(col, val, x, base, kWave, len);
+    const int64_t slot = chunk * kWave + lane;
+    if (UNPERM) {
+        const int32_t r = slot_row[slot];
+        if (r < n_rows) y[r] = acc;
+    } else {
+        __builtin_nontemporal_store(acc, y + slot);
+    }
+}
+
+// Any C: one lane per slot.
+template <int VT, bool UNPERM>
+__global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
+    int64_t C, int64_t n_chunks, int64_t n_rows,
+    const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ chunk_len,
+    const int32_t *__restrict__ col,
+    const typename vt<VT>::val_t *__restrict__ val,
+    const int32_t *__restrict__ slot_row,
+    const typename vt<VT>::vec_t *__restrict__ x,
+    typename vt<VT>::vec_t *__restrict__ y) {
+    const int64_t slot =
+        static_cast<int64_t>(blockIdx.x) * kScsThreads + threadIdx.x;
+    if (slot >= n_chunks * C) return;
+    const int64_t chunk = slot / C, i = slot - chunk * C;
+    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
+    const auto acc =
+        slot_chain<VT, kScsUnroll>(col, val, x, base, C, chunk_len[chunk]);
+    if (UNPERM) {
+        const int32_t r = slot_row[slot];
+        if (r < n_rows) y[r] = acc;
+    } else {
+        y[slot] = acc;
+    }
+}
+
+// SpMM, X and Y row-major with k columns: thread (slot, v), v fastest, so k
+// consecutive lanes share a slot's val/col (one broadcast load) and read a
+// contiguous row of X.  Y in original row order.
+template <int VT>
+__global__ __launch_bounds__(kScsThreads) void scs_spmm_kernel(
+    int64_t C, int64_t n_chunks, int64_t n_rows, int64_t k,
+    const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ chunk_len,
+    const int32_t *__restrict__ col,
+    const typename vt<VT>::val_t *__restrict__ val,
+    const int32_t *__restrict__ slot_row,
+    const typename vt<VT>::vec_t *__restrict__ X,
+    typename vt<VT>::vec_t *__restrict__ Y) {
+    using T = vt<VT>;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kScsThreads + threadIdx.x;
+    const int64_t slot = t / k, v = t - slot * k;
+    if (slot >= n_chunks * C) return;
+    const int32_t r = slot_row[slot];
+    if (r >= n_rows) return;
+    const int64_t chunk = slot / C, i = slot - chunk * C;
+    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
+    const int len = chunk_len[chunk];
+    typename T::vec_t acc = 0;
+    int j = 0;
+    constexpr int U = 4;
+    for (; j + U <= len; j += U) {
+        int32_t c[U];
+        typename T::val_t w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = col[base + (j + u) * C];
+            w[u] = val[base + (j + u) * C];
+        }
+        typename T::vec_t xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = X[static_cast<int64_t>(c[u]) * k + v];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = T::fma_(T::widen(w[u]), xv[u], acc);
+    }
+    for (; j < len; ++j) {
+        const int64_t o = base + static_cast<int64_t>(j) * C;
+        acc = T::fma_(T::widen(val[o]), X[static_cast<int64_t>(col[o]) * k + v],
+                      acc);
+    }
+    Y[static_cast<int64_t>(r) * k + v] = acc;
+}
+
+template <int VT>
+int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
+                    int64_t n_rows, const int32_t *chunk_ptr,
+                    const int32_t *chunk_len, const int32_t *col,
+                    const void *val, const int32_t *slot_row, const void *x,
+                    void *y, bool unperm) {
+    using T = vt<VT>;
+    auto v = static_cast<const typename T::val_t *>(val);
+    auto xv = static_cast<const typename T::vec_t *>(x);
+    auto yv = static_cast<typename T::vec_t *>(y);
+    if (C == kWave) {
+        dim3 grid = grid_for(n_chunks, kScsThreads / kWave);
+        if (unperm)
+            hipLaunchKernelGGL((scs_spmv_w64<VT, true>), grid, dim3(kScsThreads),
+                               0, ctx->stream, n_chunks, n_rows, chunk_ptr,
+                               chunk_len, col, v, slot_row, xv, yv);
+        else
+            hipLaunchKernelGGL((scs_spmv_w64<VT, false>), grid,
+                               dim3(kScsThreads), 0, ctx->stream, n_chunks,
+                               n_rows, chunk_ptr, chunk_len, col, v, slot_row,
+                               xv, yv);
+        SMAX_HIP_LAUNCHED(ctx, "scs_spmv_w64");
+    } else {
+        dim3 grid = grid_for(n_chunks * C, kScsThreads);
+        if (unperm)
+            hipLaunchKernelGGL((scs_spmv_generic<VT, true>), grid,
+                               dim3(kScsThreads), 0, ctx->stream, C, n_chunks,
+                               n_rows, chunk_ptr, chunk_len, col, v, slot_row,
+                               xv, yv);
+        else
+            hipLaunchKernelGGL((scs_spmv_generic<VT, false>), grid,
+                               dim3(kScsThreads), 0, ctx->stream, C, n_chunks,
+                               n_rows, chunk_ptr, chunk_len, col, v, slot_row,
+                               xv, yv);
+        SMAX_HIP_LAUNCHED(ctx, "scs_spmv_generic");
+    }
+    return SMAX_HIP_SUCCESS;
+}
+
+static int check_scs_args(smax_hip_ctx ctx, const char *fn, int value_type,
+                          int64_t C, int64_t n_chunks, const void *chunk_ptr,
+                          const void *chunk_len, const void *col,
+                          const void *val, const void *x, const void *y) {
+    if (!valid_value_type(value_type))
+        return fail(ctx, SMAX_HIP_ERROR_UNSUPPORTED, "%s: value type %d", fn,
+                    value_type);
+    SMAX_HIP_REQUIRE(ctx, C > 0 && n_chunks > 0, "%s: bad C/n_chunks", fn);
+    SMAX_HIP_REQUIRE(ctx, n_chunks * C <= INT32_MAX, "%s: too many slots", fn);
+    SMAX_HIP_REQUIRE(ctx, chunk_ptr && chunk_len && col && val && x && y,
+                     "%s: null pointer", fn);
+    return SMAX_HIP_SUCCESS;
+}
+
+}  // namespace smaxhip
+
+using namespace smaxhip;
+
+extern "C" {
+
+int smax_hip_scs_spmv(smax_hip_ctx ctx, int value_type, int64_t C,
+                      int64_t n_chunks, const int32_t *chunk_ptr,
+                      const int32_t *chunk_lengths, const int32_t *col,
+                      const void *val, const void *x, void *y) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    int rc = check_scs_args(ctx, "smax_hip_scs_spmv", value_type, C, n_chunks,
+                            chunk_ptr, chunk_lengths, col, val, x, y);
+    if (rc) return rc;
+    SMAX_HIP_DISPATCH_VT(value_type,
+                         return launch_scs_spmv<VT>(ctx, C, n_chunks,
+                                                    n_chunks * C, chunk_ptr,
+                                                    chunk_lengths, col, val,
+                                                    nullptr, x, y, false));
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+int smax_hip_scs_spmv_unpermuted(smax_hip_ctx ctx, int value_type, int64_t C,
+                                 int64_t n_chunks, int64_t n_rows,
+                                 const int32_t *chunk_ptr,
+                                 const int32_t *chunk_lengths,
+                                 const int32_t *col, const void *val,
+                                 const int32_t *slot_row, const void *x,
+                                 void *y) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    int rc = check_scs_args(ctx, "smax_hip_scs_spmv_unpermuted", value_type, C,
+                            n_chunks, chunk_ptr, chunk_lengths, col, val, x, y);
+    if (rc) return rc;
+    SMAX_HIP_REQUIRE(ctx, slot_row && n_rows > 0 && n_rows <= n_chunks * C,
+                     "smax_hip_scs_spmv_unpermuted: bad slot_row / n_rows");
+    SMAX_HIP_DISPATCH_VT(value_type,
+                         return launch_scs_spmv<VT>(ctx, C, n_chunks, n_rows,
+                                                    chunk_ptr, chunk_lengths,
+                                                    col, val, slot_row, x, y,
+                                                    true));
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+int smax_hip_scs_spmm(smax_hip_ctx ctx, int value_type, int64_t C,
+                      int64_t n_chunks, int64_t n_rows,
+                      const int32_t *chunk_ptr, const int32_t *chunk_lengths,
+                      const int32_t *col, const void *val,
+                      const int32_t *slot_row, int64_t k, const void *X,
+                      void *Y) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    int rc = check_scs_args(ctx, "smax_hip_scs_spmm", value_type, C, n_chunks,
+                            chunk_ptr, chunk_lengths, col, val, X, Y);
+    if (rc) return rc;
+    SMAX_HIP_REQUIRE(ctx, slot_row && k > 0 && n_rows > 0 &&
+                              n_rows <= n_chunks * C,
+                     "smax_hip_scs_spmm: bad slot_row / n_rows / k");
+    const int64_t threads = n_chunks * C * k;
+    SMAX_HIP_DISPATCH_VT(value_type, {
+        using T = vt<VT>;
+        hipLaunchKernelGGL((scs_spmm_kernel<VT>), grid_for(threads, kScsThreads),
+                           dim3(kScsThreads), 0, ctx->stream, C, n_chunks,
+                           n_rows, k, chunk_ptr, chunk_lengths, col,
+                           static_cast<const typename T::val_t *>(val),
+                           slot_row,
+                           static_cast<const typename T::vec_t *>(X),
+                           static_cast<typename T::vec_t *>(Y));
+        SMAX_HIP_LAUNCHED(ctx, "scs_spmm_kernel");
+        return SMAX_HIP_SUCCESS;
+    });
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+"""GPU CRS SpMV (lane-per-row and LDS row-block schedules) and CRS SpMM
+against the oracle: every output bit-identical to the reference's fma chain."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from smaxkernels_amd import capi, matgen
+from tests.helpers import almost_equal, arr, assert_bitwise, reference_kats
+
+pytestmark = pytest.mark.gpu
+VTS = [capi.F64, capi.F32, capi.BF16]
+
+
+def operands(vt, A, k=1, seed=21):
+    if vt == capi.F64:
+        return A.val, matgen.uniform_f64(seed, A.n_cols * k)
+    if vt == capi.F32:
+        return A.val.astype(np.float32), matgen.uniform_f32(seed, A.n_cols * k)
+    return matgen.to_bf16(A.val), matgen.uniform_f32(seed, A.n_cols * k)
+
+
+def run_spmv(ctx, vt, A, val, x, blocked):
+    rp, col, v, dx = (ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(val),
+                      ctx.upload(x))
+    dy = ctx.zeros(A.n_rows, x.dtype)
+    plan = ctx.crs_plan(A.n_rows, A.nnz, rp) if blocked else None
+    ctx.crs_spmv(vt, A.n_rows, A.n_cols, A.nnz, rp, col, v, dx, dy, plan)
+    y = ctx.download(dy)
+    if plan:
+        ctx.crs_plan_destroy(plan)
+    for d in (rp, col, v, dx, dy):
+        d.free()
+    return y
+
+
+class Csr:
+    def __init__(self, n_rows, n_cols, rp, col, val):
+        self.n_rows, self.n_cols = n_rows, n_cols
+        self.row_ptr, self.col, self.val = rp, col, val
+
+    @property
+    def nnz(self):
+        return int(self.row_ptr[-1])
+
+
+def test_kat_spmv_on_gpu(ctx):
+    k = reference_kats()["spmv_test.hpp"]
+    A = Csr(3, 3, arr(k, "A->row_ptr", np.int32), arr(k, "A->col", np.int32),
+            arr(k, "A->val", np.float64))
+    for blocked in (False, True):
+        y = run_spmv(ctx, capi.F64, A, A.val, np.ones(3), blocked)
+        assert almost_equal(y, arr(k, "y_expected", np.float64)).all()
+
+
+@pytest.mark.parametrize("vt", VTS)
+@pytest.mark.parametrize("blocked", [False, True])
+def test_crs_spmv_lomax_bit_exact(ctx, vt, blocked):
+    A = matgen.lomax_band(60000, half=7500)
+    val, x = operands(vt, A)
+    assert_bitwise(run_spmv(ctx, vt, A, val, x, blocked),
+                   oracle.crs_spmv(A.row_ptr, A.col, val, x))
+
+
+@pytest.mark.parametrize("blocked", [False, True])
+def test_crs_spmv_config1_poisson(ctx, blocked):
+    A = matgen.poisson2d(512)
+    x = matgen.uniform_f64(matgen.SEED_X, A.n_cols)
+    assert_bitwise(run_spmv(ctx, capi.F64, A, A.val, x, blocked),
+                   oracle.crs_spmv(A.row_ptr, A.col, A.val, x))
+
+
+@pytest.mark.parametrize("vt", VTS)
+def test_crs_spmv_long_and_empty_rows(ctx, vt):
+    """Rows across the 512-long-row and 1024-tile boundaries, empty rows,
+    a dense row, a single-row matrix and an all-empty matrix."""
+    rng = np.random.default_rng(9)
+    n = 4000
+    L = rng.integers(0, 20, n)
+    L[rng.random(n) < 0.2] = 0
+    L[[5, 6, 7]] = [511, 512, 513]
+    L[[100, 1000]] = [1023, 1025]
+    L[2000] = n
+    L[3000:3300] = 0
+    rp = np.concatenate([[0], np.cumsum(L)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in L]
+                         ).astype(np.int32)
+    A = Csr(n, n, rp, col, rng.uniform(-1, 1, rp[-1]))
+    val, x = operands(vt, A)
+    for blocked in (False, True):
+        assert_bitwise(run_spmv(ctx, vt, A, val, x, blocked),
+                       oracle.crs_spmv(A.row_ptr, A.col, val, x), f"blocked={blocked}")
+    one = Csr(1, n, np.array([0, 3000], np.int32),
+              np.arange(0, 3000, dtype=np.int32), rng.uniform(-1, 1, 3000))
+    val1, x1 = operands(vt, one)
+    for blocked in (False, True):
+        assert_bitwise(run_spmv(ctx, vt, one, val1, x1, blocked),
+                       oracle.crs_spmv(one.row_ptr, one.col, val1, x1))
+    Z = Csr(300, 300, np.zeros(301, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    valz, xz = operands(vt, Z)
+    for blocked in (False, True):
+        assert not run_spmv(ctx, vt, Z, valz, xz, blocked).any()
+
+
+@pytest.mark.parametrize("vt", VTS)
+@pytest.mark.parametrize("k", [1, 4, 8, 32])
+@pytest.mark.parametrize("row_major", [True, False])
+def test_crs_spmm_bit_exact(ctx, vt, k, row_major):
+    A = matgen.lomax_band(20000, half=2500)
+    val, X = operands(vt, A, k)
+    rp, col, v, dX = (ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(val),
+                      ctx.upload(X))
+    dY = ctx.zeros(A.n_rows * k, X.dtype)
+    ld = k if row_major else A.n_rows
+    ctx.crs_spmm(vt, A.n_rows, A.n_cols, A.nnz, rp, col, v, k, row_major, dX, ld,
+                 dY, ld)
+    assert_bitwise(ctx.download(dY),
+                   oracle.crs_spmm(A.row_ptr, A.col, val, k, row_major, X))
+
+
+def test_kat_spmm_on_gpu(ctx):
+    kats = reference_kats()
+    for name, row_major in (("spmm_test.hpp", False), ("spmm_rowmaj_test.hpp", True)):
+        k = kats[name]
+        nv = k["consts"]["n_vectors"]["value"]
+        rp, col, v = (ctx.upload(arr(k, "A->row_ptr", np.int32)),
+                      ctx.upload(arr(k, "A->col", np.int32)),
+                      ctx.upload(arr(k, "A->val", np.float64)))
+        dX, dY = ctx.upload(np.ones(3 * nv)), ctx.zeros(3 * nv, np.float64)
+        ld = nv if row_major else 3
+        ctx.crs_spmm(capi.F64, 3, 3, 5, rp, col, v, nv, row_major, dX, ld, dY, ld)
+        assert almost_equal(ctx.download(dY), arr(k, "Y_expected", np.float64)).all()
