@@ -310,11 +310,13 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
 typedef struct {
     double rel_tol;         /* stop when ||b - A x||_2 / ||b||_2 <= rel_tol */
     int64_t max_iters;      /* total (inner) iteration cap                   */
-    int64_t check_every;    /* host-side convergence check period (>= 1)     */
+    int64_t check_every;    /* iterations enqueued between host checks; the
+                               device stops itself on convergence (>= 1)     */
     double inner_rel_tol;   /* IR-CG: inner solve residual reduction         */
     int64_t max_inner;      /* IR-CG: iteration cap per inner solve          */
     int32_t inner_value_type; /* IR-CG: SMAX_HIP_F32 or SMAX_HIP_BF16        */
-    int32_t keep_diag_f32;  /* IR-CG: keep the diagonal in an f32 side array */
+    int32_t keep_diag_f32;  /* IR-CG, BF16 storage: apply the diagonal from an
+                               FP32 side array instead of the BF16 values     */
 } smax_hip_cg_params;
 
 typedef struct {

@@ -2,6 +2,7 @@
 
   _build/libsmax_hip.so     HIP kernels + C ABI (include/smax_hip.h), gfx950
   _build/libsmax_matgen.so  seeded workload generators (host C)
+  _build/libsmaxkernels_hip.so  C++ SMAX::Interface with the HIP platform
 
 Run `python -m smaxkernels_amd.build` (or call build_all()).  Every call
 recompiles; objects are compiled in parallel.
@@ -89,9 +90,26 @@ def build_matgen() -> Path:
     return lib
 
 
+def build_interface() -> Path:
+    """The C++ SMAX::Interface with the HIP platform (src/), linked against
+    libsmax_hip.so."""
+    OUT.mkdir(exist_ok=True)
+    lib = OUT / "libsmaxkernels_hip.so"
+    tmp = OUT / "libsmaxkernels_hip.so.tmp"
+    cxx = shutil.which("g++") or "c++"
+    src = PKG / "src"
+    _run([cxx, "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-shared",
+          f"-I{INCLUDE}", f"-I{src}", str(src / "interface.cpp"),
+          str(src / "error_handler.cpp"), "-o", str(tmp), f"-L{OUT}",
+          "-lsmax_hip", f"-Wl,-rpath,{OUT}"])
+    os.replace(tmp, lib)
+    return lib
+
+
 def build_all() -> None:
     build_matgen()
     build_hip()
+    build_interface()
 
 
 if __name__ == "__main__":

@@ -168,6 +168,34 @@ __global__ __launch_bounds__(256) void scs_sddmm_kernel(
     if (s >= 0 && lane == 0) dvals[s] = acc;
 }
 
+template <int VT, int G>
+static void launch_sddmm_g(smax_hip_ctx ctx, const smax_hip_scs_info &in,
+                           int64_t k, const void *Gm, const void *X, void *dv) {
+    using T = vt<VT>;
+    hipLaunchKernelGGL((scs_sddmm_kernel<VT, G>), grid_for(in.n_elements * G, 256),
+                       dim3(256), 0, ctx->stream, in.n_elements, in.C,
+                       in.n_chunks, k, in.chunk_ptr, in.col, in.slot_row,
+                       in.src_index, static_cast<const typename T::vec_t *>(Gm),
+                       static_cast<const typename T::vec_t *>(X),
+                       static_cast<typename T::vec_t *>(dv));
+}
+
+template <int VT>
+static int launch_sddmm(smax_hip_ctx ctx, const smax_hip_scs_info &in, int64_t k,
+                        int g, const void *Gm, const void *X, void *dv) {
+    switch (g) {
+    case 1: launch_sddmm_g<VT, 1>(ctx, in, k, Gm, X, dv); break;
+    case 2: launch_sddmm_g<VT, 2>(ctx, in, k, Gm, X, dv); break;
+    case 4: launch_sddmm_g<VT, 4>(ctx, in, k, Gm, X, dv); break;
+    case 8: launch_sddmm_g<VT, 8>(ctx, in, k, Gm, X, dv); break;
+    case 16: launch_sddmm_g<VT, 16>(ctx, in, k, Gm, X, dv); break;
+    case 32: launch_sddmm_g<VT, 32>(ctx, in, k, Gm, X, dv); break;
+    default: launch_sddmm_g<VT, 64>(ctx, in, k, Gm, X, dv); break;
+    }
+    SMAX_HIP_LAUNCHED(ctx, "scs_sddmm_kernel");
+    return SMAX_HIP_SUCCESS;
+}
+
 }  // namespace smaxhip
 
 using namespace smaxhip;
@@ -270,28 +298,8 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
     if (in.n_elements == 0) return SMAX_HIP_SUCCESS;
     int g = 1;
     while (g < k && g < 64) g <<= 1;
-    const int64_t threads = in.n_elements * g;
-    SMAX_HIP_DISPATCH_VT(value_type, {
-        using T = vt<VT>;
-        auto Gp = static_cast<const typename T::vec_t *>(G);
-        auto Xp = static_cast<const typename T::vec_t *>(X);
-        auto Dp = static_cast<typename T::vec_t *>(dvals);
-        const dim3 grid = grid_for(threads, 256);
-#define SDDMM_CASE(GG)                                                         \
-    case GG:                                                                   \
-        hipLaunchKernelGGL((scs_sddmm_kernel<VT, GG>), grid, dim3(256), 0,     \
-                           ctx->stream, in.n_elements, in.C, in.n_chunks, k,   \
-                           in.chunk_ptr, in.col, in.slot_row, in.src_index,    \
-                           Gp, Xp, Dp);                                        \
-        break;
-        switch (g) {
-            SDDMM_CASE(1) SDDMM_CASE(2) SDDMM_CASE(4) SDDMM_CASE(8)
-            SDDMM_CASE(16) SDDMM_CASE(32) SDDMM_CASE(64)
-        }
-#undef SDDMM_CASE
-        SMAX_HIP_LAUNCHED(ctx, "scs_sddmm_kernel");
-        return SMAX_HIP_SUCCESS;
-    });
+    SMAX_HIP_DISPATCH_VT(value_type,
+                         return launch_sddmm<VT>(ctx, in, k, g, G, X, dvals));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 

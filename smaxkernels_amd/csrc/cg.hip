@@ -1,0 +1,672 @@
+// Conjugate gradients on a SELL-C-sigma matrix: plain FP64 CG and mixed-
+// precision iterative refinement (FP64 outer residual / update / norms, inner
+// CG on FP32- or BF16-stored values with FP32 vectors and accumulation).
+//
+// Vectors live in "slot space" (length n_rows_padded, SELL order): the
+// solver uses the symmetric permutation P A P^T, whose column indices
+// col_slot[e] = perm[col[e]] are built once; every SpMV then writes its
+// output coalesced and all vector kernels are contiguous streams.  Padding
+// slots have empty rows and stay zero.  b and x are gathered/scattered
+// through perm at entry and exit.
+//
+// BF16 storage rounds the diagonal of the 27-point operator (26.01 -> 26.0)
+// and stalls refinement; with keep_diag_f32 the diagonal is taken out of the
+// BF16 array (stored as zero) and applied from an FP32 side array as
+// y_s += d_s x_s after the row's chain.
+//
+// Iterations run in batches without host round trips: alpha, beta and the
+// dot products stay on the device, the reduction that produces r.r raises a
+// device flag once sqrt(r.r) <= stop, and every later kernel of the batch
+// returns at once.  Every dot product is a fixed two-pass reduction
+// (per-block tree, then one block summing the partials in index order), so a
+// solve is deterministic for a given matrix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "convert_internal.hpp"
+
+namespace smaxhip {
+
+constexpr int kCgThreads = 256;
+constexpr int64_t kUpdBlocks = 4096;  // fixed grid of the vector kernels
+
+// Device control words of one CG loop.
+enum : int { CTL_DONE = 0, CTL_ITERS = 1 };
+
+template <typename W>
+__device__ __forceinline__ W block_sum(W v) {
+    __shared__ W ws[kCgThreads / kWave];
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    const int wave = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[wave] = v;
+    __syncthreads();
+    W t = ws[0];
+#pragma unroll
+    for (int w = 1; w < kCgThreads / kWave; ++w) t += ws[w];
+    __syncthreads();
+    return t;
+}
+
+template <typename V> struct widen_t;
+template <> struct widen_t<double> {
+    __device__ static double f(double v) { return v; }
+};
+template <> struct widen_t<float> {
+    __device__ static float f(float v) { return v; }
+};
+template <> struct widen_t<uint16_t> {
+    __device__ static float f(uint16_t v) { return bf16_to_f32(v); }
+};
+
+__device__ __forceinline__ double fma_w(double a, double b, double c) {
+    return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float fma_w(float a, float b, float c) {
+    return __builtin_fmaf(a, b, c);
+}
+
+// (A x)[s]: sequential fma chain over the slot's SELL elements, then (DIAG)
+// the FP32 diagonal term.
+template <typename V, typename W, bool DIAG>
+__device__ __forceinline__ W apply_slot(int64_t s, int64_t C,
+                                        const int32_t *__restrict__ chunk_ptr,
+                                        const int32_t *__restrict__ chunk_len,
+                                        const int32_t *__restrict__ col,
+                                        const V *__restrict__ val,
+                                        const float *__restrict__ diag,
+                                        const W *__restrict__ x) {
+    const int64_t c = s / C;
+    const int64_t base = static_cast<int64_t>(chunk_ptr[c]) + (s - c * C);
+    const int len = chunk_len[c];
+    W acc = 0;
+    constexpr int U = 8;
+    int j = 0;
+    for (; j + U <= len; j += U) {
+        int32_t cc[U];
+        V vv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cc[u] = ld_stream(col + base + (j + u) * C);
+            vv[u] = ld_stream(val + base + (j + u) * C);
+        }
+        W xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[cc[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            acc = fma_w(static_cast<W>(widen_t<V>::f(vv[u])), xv[u], acc);
+    }
+    for (; j < len; ++j)
+        acc = fma_w(static_cast<W>(widen_t<V>::f(val[base + j * C])),
+                    x[col[base + j * C]], acc);
+    if (DIAG) acc = fma_w(static_cast<W>(diag[s]), x[s], acc);
+    return acc;
+}
+
+// q = A p, partial[block] = sum p.q
+template <typename V, typename W, bool DIAG>
+__global__ __launch_bounds__(kCgThreads) void k_spmv_dot(
+    int64_t n, int64_t C, const int32_t *__restrict__ chunk_ptr,
+    const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ col,
+    const V *__restrict__ val, const float *__restrict__ diag,
+    const W *__restrict__ p, W *__restrict__ q, W *__restrict__ partial,
+    const int64_t *__restrict__ ctl) {
+    if (ctl[CTL_DONE]) return;
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+    W prod = 0;
+    if (s < n) {
+        const W v = apply_slot<V, W, DIAG>(s, C, chunk_ptr, chunk_len, col, val,
+                                           diag, p);
+        q[s] = v;
+        prod = p[s] * v;
+    }
+    const W t = block_sum(prod);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// r = b - A x, partial = sum r.r  (FP64 outer residual)
+__global__ __launch_bounds__(kCgThreads) void k_residual(
+    int64_t n, int64_t C, const int32_t *__restrict__ chunk_ptr,
+    const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ b,
+    const double *__restrict__ x, double *__restrict__ r,
+    double *__restrict__ partial) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+    double sq = 0;
+    if (s < n) {
+        const double ax = apply_slot<double, double, false>(
+            s, C, chunk_ptr, chunk_len, col, val, nullptr, x);
+        const double v = b[s] - ax;
+        r[s] = v;
+        sq = v * v;
+    }
+    const double t = block_sum(sq);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// *out = sum of partial[0..m) in a fixed order.  With ctl, skipped once the
+// loop is done, and (stop2 >= 0) raises the done flag when *out <= stop2.
+template <typename W>
+__global__ __launch_bounds__(kCgThreads) void k_reduce(const W *__restrict__ partial,
+                                                       int64_t m, W *__restrict__ out,
+                                                       int64_t *__restrict__ ctl,
+                                                       W stop2) {
+    if (ctl && ctl[CTL_DONE]) return;
+    W s = 0;
+    for (int64_t i = threadIdx.x; i < m; i += kCgThreads) s += partial[i];
+    const W t = block_sum(s);
+    if (threadIdx.x == 0) {
+        *out = t;
+        if (ctl && stop2 >= W(0) && !(t > stop2)) ctl[CTL_DONE] = 1;
+    }
+}
+
+// x += alpha p, r -= alpha q, partial = sum r.r; alpha = scal[rr] / scal[pq]
+template <typename W>
+__global__ __launch_bounds__(kCgThreads) void k_update_xr(
+    int64_t n, const W *__restrict__ scal, int rr, int pq, W *__restrict__ x,
+    W *__restrict__ r, const W *__restrict__ p, const W *__restrict__ q,
+    W *__restrict__ partial, int64_t *__restrict__ ctl) {
+    if (ctl[CTL_DONE]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[CTL_ITERS] += 1;
+    const W alpha = scal[rr] / scal[pq];
+    W sq = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kCgThreads) {
+        x[i] = fma_w(alpha, p[i], x[i]);
+        const W v = fma_w(-alpha, q[i], r[i]);
+        r[i] = v;
+        sq = fma_w(v, v, sq);
+    }
+    const W t = block_sum(sq);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// p = r + beta p, beta = scal[rr_new] / scal[rr_old]
+template <typename W>
+__global__ __launch_bounds__(kCgThreads) void k_update_p(
+    int64_t n, const W *__restrict__ scal, int rr_new, int rr_old,
+    const W *__restrict__ r, W *__restrict__ p, const int64_t *__restrict__ ctl) {
+    if (ctl[CTL_DONE]) return;
+    const W beta = scal[rr_new] / scal[rr_old];
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kCgThreads)
+        p[i] = fma_w(beta, p[i], r[i]);
+}
+
+// p = r, partial = sum r.r
+template <typename W>
+__global__ __launch_bounds__(kCgThreads) void k_copy_dot(int64_t n,
+                                                         const W *__restrict__ r,
+                                                         W *__restrict__ p,
+                                                         W *__restrict__ partial) {
+    W sq = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kCgThreads) {
+        const W v = r[i];
+        p[i] = v;
+        sq = fma_w(v, v, sq);
+    }
+    const W t = block_sum(sq);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Inner start: r32 = r64 / ||r64||, p32 = r32, d32 = 0, partial = sum r32^2
+__global__ __launch_bounds__(kCgThreads) void k_start_inner(
+    int64_t n, const double *__restrict__ scal64, int rr,
+    const double *__restrict__ r64, float *__restrict__ r32,
+    float *__restrict__ p32, float *__restrict__ d32, float *__restrict__ partial) {
+    const double inv = 1.0 / sqrt(scal64[rr]);
+    float sq = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kCgThreads) {
+        const float v = static_cast<float>(r64[i] * inv);
+        r32[i] = v;
+        p32[i] = v;
+        d32[i] = 0.0f;
+        sq = __builtin_fmaf(v, v, sq);
+    }
+    const float t = block_sum(sq);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// x64 += ||r64|| * d32
+__global__ __launch_bounds__(kCgThreads) void k_add_correction(
+    int64_t n, const double *__restrict__ scal64, int rr,
+    const float *__restrict__ d32, double *__restrict__ x64) {
+    const double nrm = sqrt(scal64[rr]);
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kCgThreads)
+        x64[i] = __builtin_fma(nrm, static_cast<double>(d32[i]), x64[i]);
+}
+
+__global__ void k_to_slots(int64_t npad, int64_t n_rows,
+                           const int32_t *__restrict__ slot_row,
+                           const double *__restrict__ v, double *__restrict__ out) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= npad) return;
+    const int32_t r = slot_row[s];
+    out[s] = r < n_rows ? v[r] : 0.0;
+}
+
+__global__ void k_from_slots(int64_t npad, int64_t n_rows,
+                             const int32_t *__restrict__ slot_row,
+                             const double *__restrict__ v, double *__restrict__ out) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= npad) return;
+    const int32_t r = slot_row[s];
+    if (r < n_rows) out[r] = v[s];
+}
+
+__global__ void k_zero_ctl(int64_t *ctl) {
+    if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
+}
+
+// ------------------------------------------------------------ matrix copies
+__global__ void k_col_slot(int64_t m, const int32_t *__restrict__ col,
+                           const int32_t *__restrict__ perm,
+                           int32_t *__restrict__ out) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < m) out[e] = perm[col[e]];
+}
+
+__global__ void k_round_f32(int64_t m, const double *__restrict__ v,
+                            float *__restrict__ out) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < m) out[e] = static_cast<float>(v[e]);
+}
+
+// binary64 -> binary32 -> bfloat16, both roundings to nearest even
+__device__ __forceinline__ uint16_t to_bf16(double v) {
+    const uint32_t u = __float_as_uint(static_cast<float>(v));
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
+    return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__global__ void k_round_bf16(int64_t m, const double *__restrict__ v,
+                             uint16_t *__restrict__ out) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < m) out[e] = to_bf16(v[e]);
+}
+
+// Per slot: the stored diagonal entry goes to diag[s] (binary32) and is
+// zeroed in the bfloat16 copy; every other element is rounded to bfloat16.
+__global__ void k_split_bf16(int64_t npad, int64_t C,
+                             const int32_t *__restrict__ chunk_ptr,
+                             const int32_t *__restrict__ chunk_len,
+                             const int32_t *__restrict__ col_slot,
+                             const double *__restrict__ val,
+                             const int32_t *__restrict__ src,
+                             uint16_t *__restrict__ out, float *__restrict__ diag) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= npad) return;
+    const int64_t c = s / C;
+    const int64_t base = static_cast<int64_t>(chunk_ptr[c]) + (s - c * C);
+    float d = 0.0f;
+    for (int j = 0; j < chunk_len[c]; ++j) {
+        const int64_t o = base + static_cast<int64_t>(j) * C;
+        if (src[o] >= 0 && col_slot[o] == s) {
+            d = static_cast<float>(val[o]);
+            out[o] = 0;
+        } else {
+            out[o] = to_bf16(val[o]);
+        }
+    }
+    diag[s] = d;
+}
+
+// ------------------------------------------------------------ host driver
+struct Mat {
+    int64_t n, C;
+    const int32_t *chunk_ptr, *chunk_len, *col;
+};
+
+struct Work {
+    smax_hip_ctx ctx;
+    std::vector<void *> ptrs;
+    explicit Work(smax_hip_ctx c) : ctx(c) {}
+    ~Work() {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (void *p : ptrs) dev_free(p);
+    }
+    template <typename T> int get(T **p, int64_t n) {
+        void *q = nullptr;
+        int rc = dev_alloc(ctx, &q, sizeof(T) * static_cast<size_t>(n), "cg");
+        if (rc) return rc;
+        ptrs.push_back(q);
+        *p = static_cast<T *>(q);
+        return SMAX_HIP_SUCCESS;
+    }
+};
+
+template <typename T>
+static int read_words(smax_hip_ctx ctx, const T *d, T *h, int n) {
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(ctx->host_word, d, sizeof(T) * n,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) h[i] = reinterpret_cast<T *>(ctx->host_word)[i];
+    return SMAX_HIP_SUCCESS;
+}
+
+static dim3 upd_grid(int64_t n) {
+    int64_t g = ceil_div(n, kCgThreads);
+    return dim3(static_cast<unsigned>(g < kUpdBlocks ? (g > 0 ? g : 1) : kUpdBlocks));
+}
+
+// CG iterations from the current (x, r, p) with r.r in scal[0].  Stops once
+// sqrt(r.r) <= stop or after max_it iterations; the host looks at the device
+// every `batch` iterations.  Returns the iterations run and the final r.r.
+template <typename V, typename W, bool DIAG>
+static int cg_loop(smax_hip_ctx ctx, const Mat &A, const V *val, const float *diag,
+                   W *x, W *r, W *p, W *q, W *partial, W *scal, int64_t *ctl,
+                   double stop, int64_t max_it, int64_t batch, int64_t *iters,
+                   double *rr_out) {
+    const int64_t n = A.n;
+    const dim3 sg = grid_for(n, kCgThreads), ug = upd_grid(n);
+    const W stop2 = static_cast<W>(stop * stop);
+    hipLaunchKernelGGL(k_zero_ctl, dim3(1), dim3(64), 0, ctx->stream, ctl);
+    int64_t h[2] = {0, 0};
+    W rr = 0;
+    int rc = read_words(ctx, scal, &rr, 1);
+    if (rc) return rc;
+    int64_t done_it = 0;
+    bool done = !(static_cast<double>(rr) > stop * stop);
+    int par = 0;  // scal[par] holds the current r.r
+    while (!done && done_it < max_it) {
+        const int64_t nb = std::min<int64_t>(batch, max_it - done_it);
+        for (int64_t b = 0, pp = par; b < nb; ++b, pp = 1 - pp) {
+            const int cur = static_cast<int>(pp), nxt = 1 - cur;
+            hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG>), sg, dim3(kCgThreads), 0,
+                               ctx->stream, n, A.C, A.chunk_ptr, A.chunk_len, A.col,
+                               val, diag, p, q, partial, ctl);
+            hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kCgThreads), 0,
+                               ctx->stream, partial, static_cast<int64_t>(sg.x),
+                               scal + 2, ctl, W(-1));
+            hipLaunchKernelGGL((k_update_xr<W>), ug, dim3(kCgThreads), 0, ctx->stream,
+                               n, scal, cur, 2, x, r, p, q, partial, ctl);
+            hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kCgThreads), 0,
+                               ctx->stream, partial, static_cast<int64_t>(ug.x),
+                               scal + nxt, ctl, stop2);
+            hipLaunchKernelGGL((k_update_p<W>), ug, dim3(kCgThreads), 0, ctx->stream,
+                               n, scal, nxt, cur, r, p, ctl);
+        }
+        SMAX_HIP_LAUNCHED(ctx, "cg iteration");
+        rc = read_words(ctx, ctl, h, 2);
+        if (rc) return rc;
+        done_it = h[CTL_ITERS];
+        par = static_cast<int>(done_it & 1);
+        rc = read_words(ctx, scal + par, &rr, 1);
+        if (rc) return rc;
+        done = h[CTL_DONE] != 0 || !(rr == rr);
+    }
+    *iters = done_it;
+    *rr_out = static_cast<double>(rr);
+    // leave the current r.r in scal[0] for the caller
+    if (par == 1)
+        SMAX_HIP_CALL(ctx, hipMemcpyAsync(scal, scal + 1, sizeof(W),
+                                          hipMemcpyDeviceToDevice, ctx->stream));
+    return SMAX_HIP_SUCCESS;
+}
+
+// FP64 residual r = b - A x; ||r||^2 lands in scal[idx] and on the host.
+static int residual(smax_hip_ctx ctx, const Mat &A, const double *val,
+                    const double *b, const double *x, double *r, double *partial,
+                    double *scal, int idx, double *rr) {
+    const dim3 sg = grid_for(A.n, kCgThreads);
+    hipLaunchKernelGGL(k_residual, sg, dim3(kCgThreads), 0, ctx->stream, A.n, A.C,
+                       A.chunk_ptr, A.chunk_len, A.col, val, b, x, r, partial);
+    hipLaunchKernelGGL((k_reduce<double>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+                       partial, static_cast<int64_t>(sg.x), scal + idx, nullptr, -1.0);
+    SMAX_HIP_LAUNCHED(ctx, "k_residual");
+    return read_words(ctx, scal + idx, rr, 1);
+}
+
+// ||v||^2 of a device vector into scal[idx] (and the host).
+template <typename W>
+static int norm2(smax_hip_ctx ctx, int64_t n, const W *v, W *tmp, W *partial, W *scal,
+                 int idx, W *out) {
+    hipLaunchKernelGGL((k_copy_dot<W>), upd_grid(n), dim3(kCgThreads), 0, ctx->stream,
+                       n, v, tmp, partial);
+    hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+                       partial, static_cast<int64_t>(upd_grid(n).x), scal + idx,
+                       nullptr, W(-1));
+    SMAX_HIP_LAUNCHED(ctx, "norm2");
+    return read_words(ctx, scal + idx, out, 1);
+}
+
+static int prepare(smax_hip_ctx ctx, smax_hip_scs A, int lo_type, bool want_diag,
+                   double *setup_s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const smax_hip_scs_info &in = A->info;
+    const int64_t m = in.n_elements > 0 ? in.n_elements : 1;
+    int rc = 0;
+    if (in.sigma > 1 && !A->col_slot) {
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->col_slot), 4 * m, "cg col_slot");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_col_slot, grid_for(m, 256), dim3(256), 0, ctx->stream,
+                           in.n_elements, in.col, in.perm, A->col_slot);
+        SMAX_HIP_LAUNCHED(ctx, "k_col_slot");
+    }
+    const int32_t *cs = A->col_slot ? A->col_slot : in.col;
+    const double *v64 = static_cast<const double *>(in.val);
+    if (lo_type == SMAX_HIP_F32 && !A->val_f32) {
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->val_f32), 4 * m, "cg f32");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_round_f32, grid_for(m, 256), dim3(256), 0, ctx->stream,
+                           in.n_elements, v64, A->val_f32);
+        SMAX_HIP_LAUNCHED(ctx, "k_round_f32");
+    }
+    if (lo_type == SMAX_HIP_BF16 && (!A->val_bf16 || A->bf16_split != (want_diag ? 1 : 0))) {
+        if (!A->val_bf16) {
+            rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->val_bf16), 2 * m, "cg bf16");
+            if (rc) return rc;
+        }
+        if (want_diag) {
+            if (!A->diag_f32) {
+                rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->diag_f32),
+                               4 * in.n_rows_padded, "cg diag");
+                if (rc) return rc;
+            }
+            hipLaunchKernelGGL(k_split_bf16, grid_for(in.n_rows_padded, 256), dim3(256),
+                               0, ctx->stream, in.n_rows_padded, in.C, in.chunk_ptr,
+                               in.chunk_lengths, cs, v64, in.src_index, A->val_bf16,
+                               A->diag_f32);
+            SMAX_HIP_LAUNCHED(ctx, "k_split_bf16");
+        } else {
+            hipLaunchKernelGGL(k_round_bf16, grid_for(m, 256), dim3(256), 0, ctx->stream,
+                               in.n_elements, v64, A->val_bf16);
+            SMAX_HIP_LAUNCHED(ctx, "k_round_bf16");
+        }
+        A->bf16_split = want_diag ? 1 : 0;
+    }
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    *setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                   .count();
+    return SMAX_HIP_SUCCESS;
+}
+
+static int check_solver_args(smax_hip_ctx ctx, const char *fn, smax_hip_scs A,
+                             const double *b, double *x,
+                             const smax_hip_cg_params *prm,
+                             smax_hip_cg_report *rep) {
+    SMAX_HIP_REQUIRE(ctx, A && b && x && prm && rep, "%s: null argument", fn);
+    SMAX_HIP_REQUIRE(ctx, A->info.val_bytes == 8,
+                     "%s: the matrix must hold FP64 values", fn);
+    SMAX_HIP_REQUIRE(ctx, A->info.n_rows == A->info.n_cols,
+                     "%s: the matrix must be square", fn);
+    SMAX_HIP_REQUIRE(ctx, prm->rel_tol > 0 && prm->max_iters > 0,
+                     "%s: rel_tol and max_iters must be positive", fn);
+    return SMAX_HIP_SUCCESS;
+}
+
+static Mat slot_matrix(smax_hip_scs A) {
+    const smax_hip_scs_info &in = A->info;
+    return Mat{in.n_rows_padded, in.C, in.chunk_ptr, in.chunk_lengths,
+               A->col_slot ? A->col_slot : in.col};
+}
+
+}  // namespace smaxhip
+
+using namespace smaxhip;
+
+extern "C" {
+
+int smax_hip_cg_f64(smax_hip_ctx ctx, smax_hip_scs A, const double *b,
+                    double *x, const smax_hip_cg_params *prm,
+                    smax_hip_cg_report *rep) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    int rc = check_solver_args(ctx, "smax_hip_cg_f64", A, b, x, prm, rep);
+    if (rc) return rc;
+    *rep = smax_hip_cg_report{};
+    rc = prepare(ctx, A, -1, false, &rep->setup_seconds);
+    if (rc) return rc;
+    const smax_hip_scs_info &in = A->info;
+    const Mat M = slot_matrix(A);
+    const double *val = static_cast<const double *>(in.val);
+    const int64_t n = M.n, batch = prm->check_every > 0 ? prm->check_every : 25;
+    Work w(ctx);
+    double *bs, *xs, *r, *p, *q, *partial, *scal;
+    int64_t *ctl;
+    rc = w.get(&bs, n) | w.get(&xs, n) | w.get(&r, n) | w.get(&p, n) |
+         w.get(&q, n) | w.get(&partial, ceil_div(n, kCgThreads) + kUpdBlocks) |
+         w.get(&scal, 8) | w.get(&ctl, 4);
+    if (rc) return SMAX_HIP_ERROR_OUT_OF_MEMORY;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(k_to_slots, grid_for(n, 256), dim3(256), 0, ctx->stream, n,
+                       in.n_rows, in.slot_row, b, bs);
+    hipLaunchKernelGGL(k_to_slots, grid_for(n, 256), dim3(256), 0, ctx->stream, n,
+                       in.n_rows, in.slot_row, x, xs);
+    double bb = 0, rr = 0;
+    rc = norm2(ctx, n, bs, q, partial, scal, 3, &bb);
+    if (rc) return rc;
+    const double bnorm = std::sqrt(bb), stop = prm->rel_tol * bnorm;
+    int64_t total = 0;
+    for (int restart = 0; restart < 4; ++restart) {
+        // (re)start from the true residual
+        rc = residual(ctx, M, val, bs, xs, r, partial, scal, 0, &rr);
+        if (rc) return rc;
+        if (bnorm == 0 || std::sqrt(rr) <= stop || total >= prm->max_iters) break;
+        hipLaunchKernelGGL((k_copy_dot<double>), upd_grid(n), dim3(kCgThreads), 0,
+                           ctx->stream, n, r, p, partial);
+        int64_t it = 0;
+        double rr_rec = 0;
+        rc = cg_loop<double, double, false>(ctx, M, val, nullptr, xs, r, p, q, partial,
+                                            scal, ctl, stop, prm->max_iters - total,
+                                            batch, &it, &rr_rec);
+        if (rc) return rc;
+        total += it;
+    }
+    hipLaunchKernelGGL(k_from_slots, grid_for(n, 256), dim3(256), 0, ctx->stream, n,
+                       in.n_rows, in.slot_row, xs, x);
+    SMAX_HIP_LAUNCHED(ctx, "k_from_slots");
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    rep->seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rep->iterations = total;
+    rep->rel_residual = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
+    if (rep->rel_residual > prm->rel_tol)
+        return fail(ctx, SMAX_HIP_ERROR_NOT_CONVERGED,
+                    "smax_hip_cg_f64: relative residual %.3e after %lld iterations",
+                    rep->rel_residual, (long long)total);
+    return SMAX_HIP_SUCCESS;
+}
+
+int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
+                  const smax_hip_cg_params *prm, smax_hip_cg_report *rep) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    int rc = check_solver_args(ctx, "smax_hip_ircg", A, b, x, prm, rep);
+    if (rc) return rc;
+    const int lo = prm->inner_value_type;
+    SMAX_HIP_REQUIRE(ctx, lo == SMAX_HIP_F32 || lo == SMAX_HIP_BF16,
+                     "smax_hip_ircg: inner_value_type must be F32 or BF16");
+    SMAX_HIP_REQUIRE(ctx, prm->inner_rel_tol > 0 && prm->inner_rel_tol < 1,
+                     "smax_hip_ircg: inner_rel_tol must lie in (0, 1)");
+    *rep = smax_hip_cg_report{};
+    // the FP32 diagonal side array applies to BF16 storage only
+    const bool diag = prm->keep_diag_f32 != 0 && lo == SMAX_HIP_BF16;
+    rc = prepare(ctx, A, lo, diag, &rep->setup_seconds);
+    if (rc) return rc;
+    const smax_hip_scs_info &in = A->info;
+    const Mat M = slot_matrix(A);
+    const double *val = static_cast<const double *>(in.val);
+    const int64_t n = M.n, batch = prm->check_every > 0 ? prm->check_every : 25;
+    const int64_t max_inner = prm->max_inner > 0 ? prm->max_inner : prm->max_iters;
+    Work w(ctx);
+    double *bs, *xs, *r64, *partial64, *scal64;
+    float *d, *r, *p, *q, *partial32, *scal32;
+    int64_t *ctl;
+    const int64_t np = ceil_div(n, kCgThreads) + kUpdBlocks;
+    rc = w.get(&bs, n) | w.get(&xs, n) | w.get(&r64, n) | w.get(&partial64, np) |
+         w.get(&scal64, 8) | w.get(&d, n) | w.get(&r, n) | w.get(&p, n) |
+         w.get(&q, n) | w.get(&partial32, np) | w.get(&scal32, 8) | w.get(&ctl, 4);
+    if (rc) return SMAX_HIP_ERROR_OUT_OF_MEMORY;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(k_to_slots, grid_for(n, 256), dim3(256), 0, ctx->stream, n,
+                       in.n_rows, in.slot_row, b, bs);
+    hipLaunchKernelGGL(k_to_slots, grid_for(n, 256), dim3(256), 0, ctx->stream, n,
+                       in.n_rows, in.slot_row, x, xs);
+    double bb = 0, rr = 0;
+    rc = norm2(ctx, n, bs, r64, partial64, scal64, 3, &bb);
+    if (rc) return rc;
+    const double bnorm = std::sqrt(bb);
+    int64_t total = 0, outer = 0;
+    double prev = HUGE_VAL;
+    for (;;) {
+        rc = residual(ctx, M, val, bs, xs, r64, partial64, scal64, 0, &rr);
+        if (rc) return rc;
+        const double rel = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
+        if (rel <= prm->rel_tol || total >= prm->max_iters || !(rel < prev)) break;
+        prev = rel;
+        ++outer;
+        hipLaunchKernelGGL(k_start_inner, upd_grid(n), dim3(kCgThreads), 0,
+                           ctx->stream, n, scal64, 0, r64, r, p, d, partial32);
+        hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+                           partial32, static_cast<int64_t>(upd_grid(n).x), scal32 + 0,
+                           nullptr, -1.0f);
+        SMAX_HIP_LAUNCHED(ctx, "k_start_inner");
+        // Solve no deeper than the outer target needs (with a 1/4 margin).
+        const double inner_tol =
+            std::max(prm->inner_rel_tol, 0.25 * prm->rel_tol / rel);
+        int64_t it = 0;
+        double rr_in = 0;
+        const int64_t cap = std::min<int64_t>(max_inner, prm->max_iters - total);
+        if (lo == SMAX_HIP_F32)
+            rc = cg_loop<float, float, false>(ctx, M, A->val_f32, nullptr, d, r, p, q,
+                                              partial32, scal32, ctl, inner_tol, cap,
+                                              batch, &it, &rr_in);
+        else if (diag)
+            rc = cg_loop<uint16_t, float, true>(ctx, M, A->val_bf16, A->diag_f32, d, r,
+                                                p, q, partial32, scal32, ctl, inner_tol,
+                                                cap, batch, &it, &rr_in);
+        else
+            rc = cg_loop<uint16_t, float, false>(ctx, M, A->val_bf16, nullptr, d, r, p,
+                                                 q, partial32, scal32, ctl, inner_tol,
+                                                 cap, batch, &it, &rr_in);
+        if (rc) return rc;
+        total += it;
+        hipLaunchKernelGGL(k_add_correction, upd_grid(n), dim3(kCgThreads), 0,
+                           ctx->stream, n, scal64, 0, d, xs);
+        SMAX_HIP_LAUNCHED(ctx, "k_add_correction");
+    }
+    hipLaunchKernelGGL(k_from_slots, grid_for(n, 256), dim3(256), 0, ctx->stream, n,
+                       in.n_rows, in.slot_row, xs, x);
+    SMAX_HIP_LAUNCHED(ctx, "k_from_slots");
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    rep->seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rep->iterations = total;
+    rep->outer_iterations = outer;
+    rep->rel_residual = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
+    if (rep->rel_residual > prm->rel_tol)
+        return fail(ctx, SMAX_HIP_ERROR_NOT_CONVERGED,
+                    "smax_hip_ircg: relative residual %.3e after %lld outer / %lld "
+                    "inner iterations",
+                    rep->rel_residual, (long long)outer, (long long)total);
+    return SMAX_HIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -10,6 +10,7 @@ struct smax_hip_scs_s {
     float *val_f32 = nullptr;     // values rounded to binary32
     uint16_t *val_bf16 = nullptr; // values rounded to bfloat16
     float *diag_f32 = nullptr;    // diagonal per slot, binary32
+    int bf16_split = -1;          // 1: val_bf16 holds 0 on the diagonal
 };
 
 namespace smaxhip {

@@ -1,0 +1,356 @@
+// Tests of the C++ SMAX::Interface with the HIP platform.
+//
+//   run_tests host   API behaviour that needs no GPU (registration, argument
+//                    checks, refused platforms and types)
+//   run_tests gpu    the reference's SpMV / SELL / BCRS / SpMM known answers
+//                    (examples/tests/*.hpp) run through PlatformType::HIP
+//
+// Exit status 0 when every selected test passes.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <algorithm>
+#include <cstdio>
+#include <functional>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "interface.hpp"
+
+using SMAX::FloatType;
+using SMAX::IntType;
+using SMAX::KernelType;
+using SMAX::PlatformType;
+
+namespace {
+
+std::map<std::string, std::function<void()>> &registry(const std::string &group) {
+    static std::map<std::string, std::map<std::string, std::function<void()>>> r;
+    return r[group];
+}
+
+struct Reg {
+    Reg(const char *group, const char *name, std::function<void()> f) {
+        registry(group)[name] = std::move(f);
+    }
+};
+
+#define TEST(group, name)                                                      \
+    static void name();                                                        \
+    static Reg reg_##name(#group, #name, name);                                \
+    static void name()
+
+void check(bool ok, const std::string &what) {
+    if (!ok) throw std::runtime_error(what);
+}
+
+// |a-b| <= 1e-14, else relative 4 eps (examples/tests/tests_common.hpp:9-35)
+template <typename T> void close(const T *a, const T *b, size_t n, const char *what) {
+    const T abs_eps = sizeof(T) == 8 ? T(1e-14) : T(1e-7);
+    const T rel = std::numeric_limits<T>::epsilon() * 4;
+    for (size_t i = 0; i < n; ++i) {
+        const T d = std::fabs(a[i] - b[i]);
+        if (d <= abs_eps || d <= rel * std::max(std::fabs(a[i]), std::fabs(b[i]))) continue;
+        throw std::runtime_error(std::string(what) + " mismatch at " + std::to_string(i) +
+                                 ": " + std::to_string(a[i]) + " vs " + std::to_string(b[i]));
+    }
+}
+
+template <typename F> void expect_throw(F f, const char *what) {
+    try {
+        f();
+    } catch (const std::exception &) {
+        return;
+    }
+    throw std::runtime_error(std::string("expected an exception: ") + what);
+}
+
+// The 3x3 matrix of examples/tests/spmv_test.hpp:14-17.
+std::vector<int> k_col = {0, 1, 1, 0, 2};
+std::vector<int> k_rp = {0, 2, 3, 5};
+std::vector<double> k_val = {1.1, 1.2, 2.2, 3.1, 3.3};
+
+} // namespace
+
+// ------------------------------------------------------------------- host
+TEST(host, refuses_platforms_not_built) {
+    SMAX::Interface smax;
+    expect_throw([&] { smax.register_kernel("a", KernelType::SPMV); }, "CPU platform");
+    expect_throw([&] { smax.register_kernel("b", KernelType::SPMV, PlatformType::CUDA); },
+                 "CUDA platform");
+    check(smax.register_kernel("c", KernelType::SPMV, PlatformType::HIP) == 0, "HIP");
+}
+
+TEST(host, missing_kernel_throws) {
+    SMAX::Interface smax;
+    expect_throw([&] { smax.kernel("nope"); }, "kernel_dne");
+}
+
+TEST(host, argument_counts_and_metadata_are_checked) {
+    SMAX::Interface smax;
+    smax.register_kernel("k", KernelType::SPMV, PlatformType::HIP);
+    auto *k = smax.kernel("k");
+    expect_throw([&] { k->register_A(3, 3, k_col.data(), k_rp.data(), k_val.data()); },
+                 "CRS with 5 args");
+    expect_throw([&] { k->register_A(-3, 3, 5, k_col.data(), k_rp.data(), k_val.data()); },
+                 "negative metadata");
+    expect_throw([&] { k->register_B(3); }, "register_B with 1 arg");
+    k->set_mat_scs(true);
+    expect_throw([&] { k->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data()); },
+                 "SCS with 6 args");
+    smax.register_kernel("m", KernelType::SPMM, PlatformType::HIP);
+    check(smax.kernel("m")->set_mat_scs(true) == 1, "SpMM has no SCS flag");
+    check(smax.kernel("m")->set_vec_row_major(true) == 0, "SpMM row-major flag");
+}
+
+TEST(host, unsupported_index_type_fails_run) {
+    SMAX::Interface smax;
+    smax.register_kernel("k", KernelType::SPMV, PlatformType::HIP, IntType::INT64);
+    std::vector<long long> col = {0, 1, 1, 0, 2}, rp = {0, 2, 3, 5};
+    std::vector<double> x(3, 1.0), y(3);
+    auto *k = smax.kernel("k");
+    k->register_A(3, 3, 5, col.data(), rp.data(), k_val.data());
+    k->register_B(3, x.data());
+    k->register_C(3, y.data());
+    check(k->run() == 1, "INT64 indices are refused on HIP");
+}
+
+// -------------------------------------------------------------------- gpu
+TEST(gpu, spmv_crs_kat) {  // examples/tests/spmv_test.hpp
+    SMAX::Interface smax;
+    std::vector<double> x(3, 1.0), y(3), want = {2.3, 2.2, 6.4};
+    for (auto impl : {SMAX::SpMVType::naive_thread_per_row, SMAX::SpMVType::naive_warp_group}) {
+        smax.register_kernel("spmv", KernelType::SPMV, PlatformType::HIP);
+        auto *k = smax.kernel("spmv");
+        k->set_kernel_implementation(impl);
+        k->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+        k->register_B(3, x.data());
+        k->register_C(3, y.data());
+        check(k->run() == 0, "run");
+        close(y.data(), want.data(), 3, "spmv_y");
+    }
+}
+
+TEST(gpu, spmv_scs_via_gpu_conversion) {
+    // spmv_scs_2_4_test_1.hpp: C=2, sigma=4 -> perm {0,2,1}; y_scs[perm] == y_crs
+    SMAX::Interface smax;
+    unsigned long long C = 2, sigma = 4, n_rows = 0, n_rows_padded = 0, n_cols = 0,
+                       n_chunks = 0, n_elements = 0, nnz = 0;
+    int *chunk_ptr = nullptr, *chunk_lengths = nullptr, *col = nullptr, *perm = nullptr;
+    double *val = nullptr;
+    smax.utils->convert_crs_to_scs<int, double, unsigned long long>(
+        3, 3, 5, k_col.data(), k_rp.data(), k_val.data(), C, sigma, n_rows,
+        n_rows_padded, n_cols, n_chunks, n_elements, nnz, chunk_ptr, chunk_lengths,
+        col, val, perm);
+    check(n_elements == 6 && n_chunks == 2 && n_rows_padded == 4, "metadata");
+    check(perm[0] == 0 && perm[1] == 2 && perm[2] == 1, "perm");
+    std::vector<double> x(3, 1.0), y(n_rows_padded), want = {2.3, 2.2, 6.4};
+    smax.register_kernel("scs", KernelType::SPMV, PlatformType::HIP);
+    auto *k = smax.kernel("scs");
+    k->set_mat_scs(true);
+    k->register_A(C, sigma, n_rows, n_rows_padded, n_cols, n_chunks, n_elements, nnz,
+                  chunk_ptr, chunk_lengths, col, val, perm);
+    k->register_B(n_cols, x.data());
+    k->register_C(n_rows_padded, y.data());
+    check(k->run() == 0, "run");
+    for (int i = 0; i < 3; ++i) close(&y[perm[i]], &want[i], 1, "y_scs[perm]");
+    delete[] chunk_ptr;
+    delete[] chunk_lengths;
+    delete[] col;
+    delete[] val;
+    delete[] perm;
+}
+
+TEST(gpu, bcrs_kat) {  // spmv_bcrs_1_1_test_3.hpp
+    SMAX::Interface smax;
+    std::vector<int> col = {0, 1, 1}, rp = {0, 2, 3};
+    std::vector<double> val = {1.1, 0.0, 0.0, 2.2, 1.3, 0.0, 1.4, 0.0, 0.0, 0.0, 0.0, 4.4};
+    std::vector<double> x(4, 1.0), y(4), want = {3.8, 2.2, 0.0, 4.4};
+    smax.register_kernel("b", KernelType::SPMV, PlatformType::HIP);
+    auto *k = smax.kernel("b");
+    k->set_mat_bcrs(true);
+    k->register_A(2, 2, 3, 2, 2, 2, 2, col.data(), rp.data(), val.data());
+    k->register_B(4, x.data());
+    k->register_C(4, y.data());
+    k->set_block_column_major(true);
+    check(k->run() == 0, "run");
+    close(y.data(), want.data(), 4, "bcrs_y");
+}
+
+TEST(gpu, bcrs_laplace_equals_crs) {  // spmv_bcrs_laplace_test.hpp, 4x8 blocks
+    const int n = 16;
+    std::vector<int> rp(1, 0), col;
+    std::vector<double> val;
+    for (int i = 0; i < n; ++i) {
+        for (int j = i - 1; j <= i + 1; ++j)
+            if (j >= 0 && j < n) {
+                col.push_back(j);
+                val.push_back(i == j ? 2.0 : -1.0);
+            }
+        rp.push_back(static_cast<int>(col.size()));
+    }
+    std::vector<double> x(n), y_crs(n), y_b(n);
+    for (int i = 0; i < n; ++i) x[i] = 1.0 / double((i + 3) % 23 + 1);
+    for (bool cm : {true, false}) {
+        SMAX::Interface smax;
+        unsigned long long nbr = 0, nbc = 0, nb = 0, bh = 0, bw = 0, hp = 0, wp = 0;
+        int *bcol = nullptr, *brp = nullptr;
+        double *bval = nullptr;
+        smax.utils->convert_crs_to_bcrs<int, double, unsigned long long>(
+            n, n, col.size(), col.data(), rp.data(), val.data(), nbr, nbc, nb, bh, bw,
+            hp, wp, bcol, brp, bval, 4, 8, 4, 8, cm);
+        smax.register_kernel("crs", KernelType::SPMV, PlatformType::HIP);
+        smax.kernel("crs")->register_A(n, n, col.size(), col.data(), rp.data(), val.data());
+        smax.kernel("crs")->register_B(n, x.data());
+        smax.kernel("crs")->register_C(n, y_crs.data());
+        check(smax.kernel("crs")->run() == 0, "crs run");
+        smax.register_kernel("b", KernelType::SPMV, PlatformType::HIP);
+        auto *k = smax.kernel("b");
+        k->set_mat_bcrs(true);
+        k->register_A(nbr, nbc, nb, bh, bw, hp, wp, bcol, brp, bval);
+        k->register_B(nbc * wp, x.data());
+        k->register_C(nbr * hp, y_b.data());
+        k->set_block_column_major(cm);
+        check(k->run() == 0, "bcrs run");
+        close(y_b.data(), y_crs.data(), n, "laplace");
+        delete[] bcol;
+        delete[] brp;
+        delete[] bval;
+    }
+}
+
+TEST(gpu, bcrs_conversion_rejects_bad_blocks) {
+    SMAX::Interface smax;
+    unsigned long long a = 0, b = 0, c = 0, d = 0, e = 0, f = 0, g = 0;
+    int *bc = nullptr, *br = nullptr;
+    double *bv = nullptr;
+    expect_throw([&] {
+        smax.utils->convert_crs_to_bcrs<int, double, unsigned long long>(
+            3, 3, 5, k_col.data(), k_rp.data(), k_val.data(), a, b, c, d, e, f, g, bc,
+            br, bv, 2, 2, 2, 2, true);
+    }, "block height must divide n_rows");
+}
+
+TEST(gpu, spmm_kats) {  // spmm_test.hpp (column-major), spmm_rowmaj_test.hpp
+    for (bool row_major : {false, true}) {
+        SMAX::Interface smax;
+        const int k = 4;
+        std::vector<double> X(3 * k, 1.0), Y(3 * k);
+        std::vector<double> want = {2.3, 2.2, 6.4, 2.3, 2.2, 6.4, 2.3, 2.2, 6.4, 2.3, 2.2, 6.4};
+        if (row_major) want = {2.3, 2.3, 2.3, 2.3, 2.2, 2.2, 2.2, 2.2, 6.4, 6.4, 6.4, 6.4};
+        smax.register_kernel("spmm", KernelType::SPMM, PlatformType::HIP);
+        auto *kk = smax.kernel("spmm");
+        kk->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+        kk->register_B(3, k, X.data());
+        kk->register_C(3, k, Y.data());
+        if (row_major) kk->set_vec_row_major(true);
+        check(kk->run() == 0, "run");
+        close(Y.data(), want.data(), 3 * k, "spmm_Y");
+    }
+}
+
+TEST(gpu, float32_and_bfloat16_values) {
+    // 2-D 5-point Laplacian on 32^2 with small-integer x: exact in every type
+    const int n = 32, N = n * n;
+    std::vector<int> rp(1, 0), col;
+    std::vector<double> v64;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const int r = i * n + j;
+            if (j > 0) { col.push_back(r - 1); v64.push_back(-1); }
+            if (i > 0) { col.push_back(r - n); v64.push_back(-1); }
+            col.push_back(r); v64.push_back(4);
+            if (i < n - 1) { col.push_back(r + n); v64.push_back(-1); }
+            if (j < n - 1) { col.push_back(r + 1); v64.push_back(-1); }
+            rp.push_back(static_cast<int>(col.size()));
+        }
+    std::vector<float> x(N), y(N), want(N), v32(v64.begin(), v64.end());
+    for (int i = 0; i < N; ++i) x[i] = float(i % 7) - 3.0f;
+    for (int r = 0; r < N; ++r) {
+        float s = 0;
+        for (int j = rp[r]; j < rp[r + 1]; ++j) s += v32[j] * x[col[j]];
+        want[r] = s;
+    }
+    std::vector<uint16_t> vbf(v64.size());
+    for (size_t i = 0; i < v64.size(); ++i) {  // -1 and 4 are exact in bfloat16
+        float f = static_cast<float>(v64[i]);
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        vbf[i] = static_cast<uint16_t>(u >> 16);
+    }
+    SMAX::Interface smax;
+    smax.register_kernel("f32", KernelType::SPMV, PlatformType::HIP, IntType::INT32,
+                         FloatType::FLOAT32);
+    auto *k = smax.kernel("f32");
+    k->register_A(N, N, col.size(), col.data(), rp.data(), v32.data());
+    k->register_B(N, x.data());
+    k->register_C(N, y.data());
+    check(k->run() == 0, "f32 run");
+    close(y.data(), want.data(), N, "f32");
+    std::fill(y.begin(), y.end(), 0.0f);
+    smax.register_kernel("bf16", KernelType::SPMV, PlatformType::HIP, IntType::INT32,
+                         FloatType::BFLOAT16);
+    k = smax.kernel("bf16");
+    k->register_A(N, N, col.size(), col.data(), rp.data(), vbf.data());
+    k->register_B(N, x.data());
+    k->register_C(N, y.data());
+    check(k->run() == 0, "bf16 run");
+    close(y.data(), want.data(), N, "bf16");
+}
+
+TEST(gpu, jacobi_style_swap_and_offsets) {
+    // x_{k+1} = A x_k three times through swap_operands (jacobi.cpp:26-58
+    // pattern), then one product into y at an offset.
+    SMAX::Interface smax;
+    std::vector<double> a(3, 1.0), b(3, 0.0), host(3, 1.0), tmp(3);
+    smax.register_kernel("it", KernelType::SPMV, PlatformType::HIP);
+    auto *k = smax.kernel("it");
+    k->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+    k->register_B(3, a.data());
+    k->register_C(3, b.data());
+    for (int it = 0; it < 3; ++it) {
+        check(k->run() == 0, "run");
+        for (int r = 0; r < 3; ++r) {
+            double s = 0;
+            for (int j = k_rp[r]; j < k_rp[r + 1]; ++j) s = std::fma(k_val[j], host[k_col[j]], s);
+            tmp[r] = s;
+        }
+        host = tmp;
+        k->swap_operands();
+    }
+    // iterations 1 and 3 wrote b, iteration 2 wrote a
+    close(b.data(), host.data(), 3, "swapped iterate");
+    // one product with x read at offset 1 and y written at offset 1
+    std::vector<double> xl = {9.0, 1.0, 2.0, 3.0}, yl(4, 0.0);
+    smax.register_kernel("off", KernelType::SPMV, PlatformType::HIP);
+    auto *o = smax.kernel("off");
+    o->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+    o->register_B(4, xl.data());
+    o->register_C(4, yl.data());
+    check(o->initialize() == 0 && o->apply(0, 1, 1) == 0, "offset apply");
+    check(o->finalize() == 0, "offset finalize");
+    // finalize copies n_rows entries from the start of device y
+    // (spmv_cuda_core.cuh:190-200): y[1..3] = A * (1,2,3), y[0] untouched
+    const double w1 = std::fma(1.2, 2.0, 1.1 * 1.0);
+    close(&yl[1], &w1, 1, "offset y[1]");
+}
+
+int main(int argc, char **argv) {
+    const std::string group = argc > 1 ? argv[1] : "host";
+    int failed = 0, passed = 0;
+    for (auto &[name, fn] : registry(group)) {
+        try {
+            fn();
+            ++passed;
+            std::printf("PASSED %s\n", name.c_str());
+        } catch (const std::exception &e) {
+            ++failed;
+            std::printf("FAILED %s: %s\n", name.c_str(), e.what());
+        }
+        std::fflush(stdout);
+    }
+    std::printf("%s: %d passed, %d failed\n", group.c_str(), passed, failed);
+    return failed ? 1 : (passed ? 0 : 2);
+}

@@ -1,0 +1,197 @@
+"""Differentiation: SELL layout of A^T, adjoint SpMV/SpMM, SDDMM value
+gradient, and a thin torch.autograd.Function over the C ABI checked against
+FP64 central finite differences."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from smaxkernels_amd import capi, matgen
+from tests.helpers import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def upload_crs(ctx, A, val=None):
+    val = A.val if val is None else val
+    return ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(val)
+
+
+@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 4096), (32, 1 << 30), (1, 1)])
+def test_transpose_layout_matches_explicit_transpose(ctx, C_, sigma):
+    A = matgen.lomax_band(20000, half=2500)
+    rp, col, v = upload_crs(ctx, A)
+    T = ctx.scs_transpose_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, 8, C_, sigma)
+    trp, tcol, tval = oracle.crs_transpose(A.n_rows, A.n_cols, A.row_ptr, A.col, A.val)
+    want = oracle.crs_to_scs(trp, tcol, tval, C_, sigma)
+    got = T.arrays(np.float64)
+    for f in ("chunk_ptr", "chunk_lengths", "col", "perm"):
+        assert_bitwise(got[f], want[f], f)
+    assert_bitwise(got["val"], want["val"], "val")
+    real = got["src_index"] >= 0
+    assert_bitwise(got["val"][real], A.val[got["src_index"][real]], "src_index")
+    # adjoint SpMV through the transpose layout == reference CRS SpMV of A^T
+    g = matgen.uniform_f64(5, A.n_rows)
+    dg, dz = ctx.upload(g), ctx.zeros(A.n_cols, np.float64)
+    T.spmv(capi.F64, dg, dz, unpermuted=True)
+    assert_bitwise(ctx.download(dz), oracle.crs_spmv(trp, tcol, tval, g), "A^T g")
+    T.destroy()
+
+
+def test_transpose_refresh_follows_new_values(ctx):
+    A = matgen.lomax_band(5000, half=600)
+    rp, col, v = upload_crs(ctx, A)
+    T = ctx.scs_transpose_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, 8, 64, 256)
+    new = matgen.uniform_f64(99, A.nnz)
+    T.refresh_values(ctx.upload(new))
+    trp, tcol, tval = oracle.crs_transpose(A.n_rows, A.n_cols, A.row_ptr, A.col, new)
+    assert_bitwise(T.arrays(np.float64)["val"],
+                   oracle.crs_to_scs(trp, tcol, tval, 64, 256)["val"])
+    T.destroy()
+
+
+@pytest.mark.parametrize("vt,k", [(capi.F64, 1), (capi.F64, 5), (capi.F64, 32),
+                                  (capi.F32, 1), (capi.F32, 32), (capi.BF16, 32)])
+def test_sddmm(ctx, vt, k):
+    A = matgen.lomax_band(20000, half=2500)
+    val = {capi.F64: A.val, capi.F32: A.val.astype(np.float32),
+           capi.BF16: matgen.to_bf16(A.val)}[vt]
+    rp, col, v = upload_crs(ctx, A, val)
+    S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, val.dtype.itemsize,
+                         64, 512)
+    vec = np.float64 if vt == capi.F64 else np.float32
+    G = matgen.uniform_f64(6, A.n_rows * k).astype(vec)
+    X = matgen.uniform_f64(7, A.n_cols * k).astype(vec)
+    dv = ctx.zeros(A.nnz, vec)
+    S.sddmm(vt, k, ctx.upload(G), ctx.upload(X), dv)
+    got = ctx.download(dv).astype(np.float64)
+    want = oracle.sddmm_f64(A.row_ptr, A.col, k, G.astype(np.float64),
+                            X.astype(np.float64))
+    rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
+    scale = np.abs(G.astype(np.float64).reshape(-1, k)[rows] *
+                   X.astype(np.float64).reshape(-1, k)[A.col]).sum(axis=1)
+    if k == 1:
+        # one product, rounded once
+        assert_bitwise(got, (G[rows] * X[A.col]).astype(np.float64), "k=1")
+    u = np.finfo(vec).eps / 2
+    bound = (np.ceil(np.log2(k)) + 1) * u * scale + 1e-300
+    assert (np.abs(got - want) <= bound).all()
+    S.destroy()
+
+
+# ------------------------------------------------------- autograd wrapper
+torch = pytest.importorskip("torch")
+
+
+class SpMMFunction(torch.autograd.Function):
+    """Y = A X for a fixed CRS pattern, differentiable in A's values and X.
+
+    forward: CRS SpMM (row-major X).  backward: dX = A^T dY through the SELL
+    layout of A^T (values refreshed from the current A), dvals = SDDMM on A's
+    SELL layout.  All three run in libsmax_hip.so through the C ABI."""
+
+    @staticmethod
+    def forward(fctx, vals, X, op):
+        k = X.shape[1]
+        Y = torch.empty(op.n_rows, k, dtype=X.dtype, device=X.device)
+        op.ctx.crs_spmm(op.vt, op.n_rows, op.n_cols, op.nnz, op.rp, op.col, vals,
+                        k, True, X, k, Y, k)
+        fctx.save_for_backward(vals, X)
+        fctx.op = op
+        return Y
+
+    @staticmethod
+    def backward(fctx, dY):
+        vals, X = fctx.saved_tensors
+        op = fctx.op
+        dY = dY.contiguous()
+        k = X.shape[1]
+        op.T.refresh_values(vals)
+        dX = torch.empty_like(X)
+        op.T.spmm(op.vt, k, dY, dX)
+        dvals = torch.empty_like(vals)
+        op.S.sddmm(op.vt, k, dY, X, dvals)
+        return dvals, dX, None
+
+
+class SpMMOp:
+    def __init__(self, ctx, A, vt):
+        self.ctx, self.vt = ctx, vt
+        self.n_rows, self.n_cols, self.nnz = A.n_rows, A.n_cols, A.nnz
+        self.rp, self.col = ctx.upload(A.row_ptr), ctx.upload(A.col)
+        v = ctx.upload(A.val.astype(np.float32))
+        self.S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, self.rp, self.col, v, 4,
+                                  64, 4096)
+        self.T = ctx.scs_transpose_from_crs(A.n_rows, A.n_cols, A.nnz, self.rp,
+                                            self.col, v, 4, 64, 4096)
+
+
+def loss_rows_f64(A, vals, X, W, rows):
+    """sum_v W[i,v] * (A X)[i,v] over the given rows, in FP64."""
+    tot = 0.0
+    for i in rows:
+        s, e = A.row_ptr[i], A.row_ptr[i + 1]
+        yi = vals[s:e].astype(np.float64) @ X[A.col[s:e]].astype(np.float64)
+        tot += float(yi @ W[i].astype(np.float64))
+    return tot
+
+
+def test_autograd_matches_central_differences(ctx):
+    """C4 at reduced N: FP32 values, k = 32, loss = sum(Y o W).  64 value and
+    64 X entries; metric |g - g_FD| / max(|g_FD|, sum_v |W_iv X_jv|) <= 1e-6.
+    L is linear in each entry, so the FP64 central difference (h = 2^-10
+    max(1,|v|)) is exact up to rounding; it is evaluated on the rows the
+    entry touches (every other term of L cancels identically)."""
+    N, k = 100_000, 32
+    A = matgen.lomax_band(N)
+    dev = torch.device("cuda", 0)
+    ctx.sync()
+    op = SpMMOp(ctx, A, capi.F32)
+    stream = torch.cuda.current_stream(dev)
+    ctx.lib.smax_hip_set_stream(ctx.handle, stream.cuda_stream)
+    try:
+        vals = torch.tensor(A.val.astype(np.float32), device=dev, requires_grad=True)
+        Xh = matgen.uniform_f32(matgen.SEED_X, N * k).reshape(N, k)
+        Wh = matgen.uniform_f32(matgen.SEED_W, N * k).reshape(N, k)
+        X = torch.tensor(Xh, device=dev, requires_grad=True)
+        W = torch.tensor(Wh, device=dev)
+        Y = SpMMFunction.apply(vals, X, op)
+        (Y * W).sum().backward()
+        torch.cuda.synchronize()
+        gv = vals.grad.cpu().numpy().astype(np.float64)
+        gx = X.grad.cpu().numpy().astype(np.float64)
+        Yh = Y.detach().cpu().numpy()
+    finally:
+        ctx.lib.smax_hip_set_stream(ctx.handle, None)
+    np.testing.assert_allclose(
+        Yh, oracle.crs_spmm(A.row_ptr, A.col, A.val.astype(np.float32), k, True,
+                            Xh.ravel()).reshape(N, k), rtol=0, atol=0)
+    rng = np.random.default_rng(2024)
+    v64 = A.val.astype(np.float32).astype(np.float64)
+    rows_of = np.repeat(np.arange(N), np.diff(A.row_ptr))
+    worst = 0.0
+    for j in rng.choice(A.nnz, 64, replace=False):
+        i = rows_of[j]
+        h = 2.0 ** -10 * max(1.0, abs(v64[j]))
+        vp, vm = v64.copy(), v64.copy()
+        vp[j] += h
+        vm[j] -= h
+        fd = (loss_rows_f64(A, vp, Xh, Wh, [i]) - loss_rows_f64(A, vm, Xh, Wh, [i])) / (2 * h)
+        scale = max(abs(fd), float(np.abs(Wh[i].astype(np.float64) *
+                                          Xh[A.col[j]].astype(np.float64)).sum()))
+        worst = max(worst, abs(gv[j] - fd) / scale)
+    cols = A.col
+    for idx in rng.choice(N * k, 64, replace=False):
+        jr, vv = divmod(int(idx), k)
+        rows = np.unique(rows_of[cols == jr])
+        h = 2.0 ** -10 * max(1.0, abs(float(Xh[jr, vv])))
+        Xp = Xh.astype(np.float64).copy()
+        Xm = Xp.copy()
+        Xp[jr, vv] += h
+        Xm[jr, vv] -= h
+        fd = (loss_rows_f64(A, v64, Xp, Wh, rows) - loss_rows_f64(A, v64, Xm, Wh, rows)) / (2 * h)
+        mask = cols == jr
+        scale = max(abs(fd), float(np.abs(v64[mask] * Wh[rows_of[mask], vv]).sum()))
+        worst = max(worst, abs(gx[jr, vv] - fd) / scale)
+    assert worst <= 1e-6, worst

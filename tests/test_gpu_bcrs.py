@@ -1,0 +1,139 @@
+"""GPU CRS -> BCRS conversion, BCRS SpMV and BCRS SpMM against the oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from smaxkernels_amd import capi, matgen
+from tests.helpers import almost_equal, arr, assert_bitwise, reference_kats
+
+pytestmark = pytest.mark.gpu
+KATS = reference_kats()
+
+
+def gpu_bcrs(ctx, n_rows, n_cols, rp, col, val, bh, bw, hp, wp, cm):
+    d = [ctx.upload(a) for a in (rp, col, val)]
+    b = ctx.bcrs_from_crs(n_rows, n_cols, int(rp[-1]), d[0], d[1], d[2],
+                          val.dtype.itemsize, bh, bw, hp, wp, cm)
+    for a in d:
+        a.free()
+    return b
+
+
+@pytest.mark.parametrize("name", ["spmv_bcrs_1_1_test_1.hpp",
+                                  "spmv_bcrs_1_1_test_2.hpp"])
+def test_kat_bcrs_conversion_on_gpu(ctx, name):
+    k = KATS[name]
+    c = {n: v["value"] for n, v in k["consts"].items()}
+    b = gpu_bcrs(ctx, 4, 4, arr(k, "A_crs->row_ptr", np.int32),
+                 arr(k, "A_crs->col", np.int32), arr(k, "A_crs->val", np.float64),
+                 c["target_b_height"], c["target_b_width"], c["target_height_pad"],
+                 c["target_width_pad"], c["use_blocked_column_major"])
+    got = b.arrays(np.float64)
+    assert b.info.n_blocks == k["ctors"]["A_bcrs_exp"]["args"][2]
+    assert_bitwise(got["col"], arr(k, "A_bcrs_exp->col", np.int32))
+    assert_bitwise(got["row_ptr"], arr(k, "A_bcrs_exp->row_ptr", np.int32))
+    assert almost_equal(got["val"], arr(k, "A_bcrs_exp->val", np.float64)).all()
+    b.destroy()
+
+
+def test_kat_bcrs_spmv_on_gpu(ctx):
+    k = KATS["spmv_bcrs_1_1_test_3.hpp"]
+    nbr, nbc, nb, bh, bw, hp, wp = k["ctors"]["A_bcrs"]["args"]
+    d = [ctx.upload(arr(k, n, t)) for n, t in (("A_bcrs->col", np.int32),
+                                               ("A_bcrs->row_ptr", np.int32),
+                                               ("A_bcrs->val", np.float64))]
+    dx, dy = ctx.upload(np.ones(nbc * wp)), ctx.zeros(nbr * hp, np.float64)
+    ctx.bcrs_spmv(capi.F64, nbr, bh, bw, hp, wp, d[0], d[1], d[2], True, dx, dy)
+    assert almost_equal(ctx.download(dy), arr(k, "y_expected", np.float64)).all()
+
+
+@pytest.mark.parametrize("cm", [True, False])
+def test_laplace_blocked_equals_crs_on_gpu(ctx, cm):
+    """spmv_bcrs_laplace_test.hpp: 16x16 1-D Laplacian, 4x8 blocks."""
+    from tests.test_oracle import laplace1d
+    rp, col, val = laplace1d(16)
+    b = gpu_bcrs(ctx, 16, 16, rp, col, val, 4, 8, 4, 8, cm)
+    x = 1.0 / (((np.arange(16) + 3) % 23) + 1.0)
+    dx, dy = ctx.upload(x), ctx.zeros(16, np.float64)
+    i = b.info
+    ctx.bcrs_spmv(capi.F64, i.n_block_rows, 4, 8, 4, 8, i.col, i.row_ptr, i.val,
+                  cm, dx, dy)
+    assert almost_equal(ctx.download(dy), oracle.crs_spmv(rp, col, val, x)).all()
+    b.destroy()
+
+
+def vals_for(vt, v64):
+    if vt == capi.F64:
+        return v64
+    if vt == capi.F32:
+        return v64.astype(np.float32)
+    return matgen.to_bf16(v64)
+
+
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("bh,bw,hp,wp,cm", [(4, 4, 4, 4, True), (4, 4, 4, 4, False),
+                                            (4, 4, 5, 6, True), (2, 4, 3, 4, False)])
+def test_block7_conversion_spmv_spmm(ctx, vt, bh, bw, hp, wp, cm):
+    A = matgen.block7(10, 4)  # 4000 rows, 4x4 dense blocks
+    val = vals_for(vt, A.val)
+    b = gpu_bcrs(ctx, A.n_rows, A.n_cols, A.row_ptr, A.col, val, bh, bw, hp, wp, cm)
+    want = oracle.crs_to_bcrs(A.n_rows, A.n_cols, A.row_ptr, A.col, val, bh, bw, hp,
+                              wp, cm)
+    got = b.arrays(val.dtype)
+    assert b.info.n_blocks == want["n_blocks"]
+    for f in ("col", "row_ptr"):
+        assert_bitwise(got[f], want[f], f)
+    assert_bitwise(got["val"].view(np.uint8), want["val"].view(np.uint8), "val")
+    i = b.info
+    xt = np.float64 if vt == capi.F64 else np.float32
+    x = matgen.uniform_f64(3, i.n_block_cols * wp).astype(xt)
+    dx, dy = ctx.upload(x), ctx.zeros(i.n_block_rows * hp, xt)
+    ctx.bcrs_spmv(vt, i.n_block_rows, bh, bw, hp, wp, i.col, i.row_ptr, i.val, cm,
+                  dx, dy)
+    yw = oracle.bcrs_spmv(i.n_block_rows, bh, bw, hp, wp, want["col"],
+                          want["row_ptr"], want["val"], cm, x)
+    rows = (np.arange(i.n_block_rows * hp) % hp) < bh
+    assert_bitwise(ctx.download(dy)[rows], yw[rows], "bcrs spmv")
+    for k in (1, 8, 64):
+        X = matgen.uniform_f64(4, i.n_block_cols * wp * k).astype(xt)
+        dX, dY = ctx.upload(X), ctx.zeros(i.n_block_rows * hp * k, xt)
+        ctx.bcrs_spmm(vt, i.n_block_rows, bh, bw, hp, wp, i.col, i.row_ptr, i.val,
+                      cm, k, dX, dY)
+        Yw = oracle.bcrs_spmm(i.n_block_rows, bh, bw, hp, wp, want["col"],
+                              want["row_ptr"], want["val"], cm, k, X)
+        rk = np.repeat(rows, k)
+        assert_bitwise(ctx.download(dY)[rk], Yw[rk], f"bcrs spmm k={k}")
+        dX.free()
+        dY.free()
+    b.destroy()
+
+
+def test_bcrs_scalar_equivalence(ctx):
+    """BCRS SpMV of the 7-point block matrix equals the CRS SpMV up to
+    summation order (rows of the block matrix padded by zeros)."""
+    A = matgen.block7(12, 4)
+    b = gpu_bcrs(ctx, A.n_rows, A.n_cols, A.row_ptr, A.col, A.val, 4, 4, 4, 4, True)
+    x = matgen.uniform_f64(8, A.n_cols)
+    dx, dy = ctx.upload(x), ctx.zeros(A.n_rows, np.float64)
+    i = b.info
+    ctx.bcrs_spmv(capi.F64, i.n_block_rows, 4, 4, 4, 4, i.col, i.row_ptr, i.val,
+                  True, dx, dy)
+    want = oracle.crs_spmv(A.row_ptr, A.col, A.val, x)
+    np.testing.assert_allclose(ctx.download(dy), want, rtol=1e-13, atol=1e-13)
+    b.destroy()
+
+
+def test_bcrs_conversion_rejects_what_the_reference_rejects(ctx):
+    A = matgen.poisson2d(6)  # 36 x 36
+    for bh, bw, hp, wp in ((5, 4, 5, 4), (4, 5, 4, 5), (4, 4, 3, 4), (4, 4, 4, 3)):
+        with pytest.raises(capi.SmaxHipError) as e:
+            gpu_bcrs(ctx, 36, 36, A.row_ptr, A.col, A.val, bh, bw, hp, wp, True)
+        assert e.value.status == capi.ERROR_INVALID_VALUE
+    # a row whose block columns descend (convert_formats.hpp:230-234 throws)
+    rp = np.array([0, 2, 2, 2, 2], np.int32)
+    col = np.array([3, 0], np.int32)
+    with pytest.raises(capi.SmaxHipError) as e:
+        gpu_bcrs(ctx, 4, 4, rp, col, np.ones(2), 2, 2, 2, 2, True)
+    assert "Blocked Column" in str(e.value)

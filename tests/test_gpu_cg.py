@@ -1,0 +1,74 @@
+"""FP64 CG and mixed-precision IR-CG on the GPU.  Every reported residual is
+re-checked on the host with the oracle's FP64 CRS SpMV."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from smaxkernels_amd import capi, matgen
+
+pytestmark = pytest.mark.gpu
+
+
+def true_rel_residual(A, b, x):
+    r = b - oracle.crs_spmv(A.row_ptr, A.col, A.val, x)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+def solve(ctx, A, C_, sigma, mixed, lo=capi.BF16, diag=True, tol=1e-10,
+          inner=1e-4, max_iters=20000):
+    rp, col, v = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
+    S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, 8, C_, sigma)
+    b = matgen.uniform_f64(matgen.SEED_B, A.n_rows)
+    db, dx = ctx.upload(b), ctx.zeros(A.n_rows, np.float64)
+    prm = capi.CgParams(rel_tol=tol, max_iters=max_iters, check_every=10,
+                        inner_rel_tol=inner, max_inner=5000,
+                        inner_value_type=lo, keep_diag_f32=int(diag))
+    rep = ctx.cg(S, db, dx, prm, mixed)
+    x = ctx.download(dx)
+    S.destroy()
+    return rep, b, x
+
+
+@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 4096), (32, 1 << 30)])
+def test_fp64_cg_reaches_1e10(ctx, C_, sigma):
+    A = matgen.stencil27(20)
+    rep, b, x = solve(ctx, A, C_, sigma, mixed=False)
+    assert rep.status == capi.SUCCESS
+    assert rep.rel_residual <= 1e-10
+    assert true_rel_residual(A, b, x) <= 1.5e-10
+    assert 10 < rep.iterations < 500
+
+
+@pytest.mark.parametrize("lo,diag", [(capi.F32, False), (capi.BF16, True),
+                                     (capi.BF16, False)])
+def test_ircg_reaches_1e10(ctx, lo, diag):
+    A = matgen.stencil27(20)
+    rep, b, x = solve(ctx, A, 64, 1, mixed=True, lo=lo, diag=diag)
+    assert rep.status == capi.SUCCESS, rep.rel_residual
+    assert rep.rel_residual <= 1e-10
+    assert true_rel_residual(A, b, x) <= 1.5e-10
+    assert rep.outer_iterations >= 2
+
+
+def test_ircg_permuted_layout(ctx):
+    """sigma > 1 reorders rows: the solver works on P A P^T in slot space."""
+    A = matgen.stencil27(16)
+    rep, b, x = solve(ctx, A, 32, 1 << 30, mixed=True, lo=capi.F32, diag=True)
+    assert rep.rel_residual <= 1e-10
+    assert true_rel_residual(A, b, x) <= 1.5e-10
+
+
+def test_solvers_agree(ctx):
+    A = matgen.stencil27(14)
+    _, b, x64 = solve(ctx, A, 64, 1, mixed=False, tol=1e-12)
+    _, _, xir = solve(ctx, A, 64, 1, mixed=True, tol=1e-12)
+    assert np.max(np.abs(x64 - xir)) <= 1e-9 * np.max(np.abs(x64))
+
+
+def test_not_converged_is_reported(ctx):
+    A = matgen.stencil27(20)
+    rep, _, _ = solve(ctx, A, 64, 1, mixed=False, max_iters=5)
+    assert rep.status == capi.ERROR_NOT_CONVERGED
+    assert rep.rel_residual > 1e-10
