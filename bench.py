@@ -95,6 +95,53 @@ def cpu_baseline(A, x, seconds) -> dict:
                       f"{2 * A.nnz / t / 1e9:.2f} GFLOP/s), OpenMP static rows"}
 
 
+def cg_section(ctx) -> dict:
+    """27-point stencil on 160^3 (N = 4,096,000, nnz = 109,215,352, diagonal
+    26.01): FP64 CG vs iterative refinement with BF16 values (FP32 diagonal
+    side array) to ||b - A x|| / ||b|| <= 1e-10, b uniform [-1, 1)."""
+    from oracle import oracle
+    log("C3: 27-point stencil 160^3")
+    A = matgen.stencil27(160)
+    b = matgen.uniform_f64(matgen.SEED_B, A.n_rows)
+    rp, col, val = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
+    S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 8, 64, 1)
+    db = ctx.upload(b)
+    out = {"n_rows": A.n_rows, "nnz": A.nnz, "format": "SELL-64-1",
+           "fill": round(S.info.n_elements / A.nnz, 4)}
+
+    def solve(mixed, lo=capi.BF16, inner=1e-6, reps=3):
+        best = None
+        for _ in range(reps):
+            dx = ctx.zeros(A.n_rows, np.float64)
+            prm = capi.CgParams(rel_tol=1e-10, max_iters=20000, check_every=25,
+                                inner_rel_tol=inner, max_inner=5000,
+                                inner_value_type=lo, keep_diag_f32=1)
+            rep = ctx.cg(S, db, dx, prm, mixed)
+            if best is None or rep.seconds < best[0].seconds:
+                best = (rep, ctx.download(dx))
+            dx.free()
+        rep, x = best
+        r = b - oracle.crs_spmv(A.row_ptr, A.col, A.val, x)
+        return {"status": rep.status, "seconds": round(rep.seconds, 5),
+                "iterations": rep.iterations, "outer": rep.outer_iterations,
+                "rel_residual_gpu": rep.rel_residual,
+                "rel_residual_oracle": float(np.linalg.norm(r) / np.linalg.norm(b)),
+                "setup_seconds": round(rep.setup_seconds, 5)}, x
+
+    fp64, x64 = solve(False)
+    bf16, xbf = solve(True, capi.BF16)
+    f32, _ = solve(True, capi.F32)
+    bf16["max_abs_diff_vs_fp64_cg"] = float(np.max(np.abs(xbf - x64)))
+    out.update({"fp64_cg": fp64, "ir_bf16": bf16, "ir_f32": f32,
+                "speedup_bf16_vs_fp64": round(fp64["seconds"] / bf16["seconds"], 3),
+                "speedup_f32_vs_fp64": round(fp64["seconds"] / f32["seconds"], 3)})
+    log(f"C3 {out}")
+    for d in (rp, col, val, db):
+        d.free()
+    S.destroy()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,7 +152,8 @@ def main() -> int:
     ap.add_argument("--full-sweep", action="store_true",
                     help="also time sigma in {1, 64} (17x padded, 13 GB each)")
     ap.add_argument("--only", default="",
-                    help="comma list of sections to run: stream,sell,crs,c1,cpu")
+                    help="comma list of sections to run: "
+                         "stream,sell,crs,c1,cg,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -203,6 +251,9 @@ def main() -> int:
             d.free()
         ps.destroy()
         log(f"C1 {extra['c1_poisson512_f64_sell64_1']}")
+
+    if want("cg"):
+        extra["c3_cg"] = cg_section(ctx)
 
     base = None
     if want("cpu"):

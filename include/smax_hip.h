@@ -212,6 +212,8 @@ typedef struct {
     int32_t *perm;          /* n_rows: perm[old] = new slot            */
     int32_t *slot_row;      /* n_rows_padded: slot -> old row (or pad) */
     int32_t *src_index;     /* n_elements: CRS nnz index, -1 = padding */
+    int32_t *slot_len;      /* n_rows_padded: stored elements of each slot's
+                               row (the sorted row lengths; 0 for padding) */
 } smax_hip_scs_info;
 
 /*
@@ -228,6 +230,18 @@ int smax_hip_scs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
                           smax_hip_scs *out);
 int smax_hip_scs_get_info(smax_hip_scs scs, smax_hip_scs_info *info);
 int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs);
+
+/*
+ * SpMV with a matrix built by smax_hip_scs_from_crs: the same result, bit for
+ * bit, as smax_hip_scs_spmv (unpermuted = 0: y in SELL order, n_rows_padded
+ * entries) or smax_hip_scs_spmv_unpermuted (unpermuted = 1).  It uses the
+ * handle's slot lengths so each lane stops at its row's last stored element
+ * (padding is neither read nor multiplied; a non-finite x[0] still reaches
+ * every padded row as the reference's 0*x[0] term would), and picks the
+ * workgroup-to-XCD schedule from sigma.  No reference counterpart.
+ */
+int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
+                       const void *x, void *y, int unpermuted);
 
 /* Re-gathers the SELL values from a CRS value array of the same pattern
  * (val_sell[e] = crs_val[src_index[e]], 0 for padding). */
@@ -263,9 +277,11 @@ int smax_hip_bcrs_spmm(smax_hip_ctx ctx, int value_type,
 typedef struct {
     int64_t n_block_rows, n_block_cols, n_blocks, b_h, b_w, h_pad, w_pad,
         val_bytes, block_col_major;
-    int32_t *col;     /* n_blocks         */
-    int32_t *row_ptr; /* n_block_rows + 1 */
-    void *val;        /* n_blocks * h_pad * w_pad, zero filled */
+    int32_t *col;       /* n_blocks         */
+    int32_t *row_ptr;   /* n_block_rows + 1 */
+    void *val;          /* n_blocks * h_pad * w_pad, zero filled */
+    int32_t *src_block; /* transposes: block of the source matrix each block
+                           copies (NULL for a conversion from CRS) */
 } smax_hip_bcrs_info;
 
 /* CRS -> BCRS on the GPU, identical to SMAX::Utils::convert_crs_to_bcrs
@@ -282,6 +298,34 @@ int smax_hip_bcrs_get_info(smax_hip_bcrs bcrs, smax_hip_bcrs_info *info);
 int smax_hip_bcrs_destroy(smax_hip_ctx ctx, smax_hip_bcrs bcrs);
 
 /* ---------------------------------------------------------- differentiation */
+/*
+ * BCRS layout of A^T built once from a BCRS A: the block pattern is
+ * transposed with a stable sort by block column (deterministic), and a
+ * column-major h x w block read row-major with the dimensions swapped is the
+ * transposed block, so A^T's blocks are A's blocks (bytes unchanged,
+ * reordered) with b_h/b_w, h_pad/w_pad and the block layout flag swapped.
+ * Applying smax_hip_bcrs_spmm to it computes A^T G.  No reference
+ * counterpart.
+ */
+int smax_hip_bcrs_transpose(smax_hip_ctx ctx, smax_hip_bcrs A,
+                            smax_hip_bcrs *out);
+
+/* Re-copies a transpose's blocks from the source matrix's current values
+ * (val of the BCRS A it was built from). */
+int smax_hip_bcrs_refresh_values(smax_hip_ctx ctx, smax_hip_bcrs T,
+                                 const void *src_val);
+
+/*
+ * SDDMM on a BCRS layout: for every block j of block row r and in-block
+ * position (ix < b_h, jx < b_w),
+ *   dvals[j*h_pad*w_pad + off(ix,jx)] =
+ *       sum_v G[(r*h_pad+ix)*k + v] * X[(col[j]*w_pad+jx)*k + v],
+ * the gradient of sum(Y o G) w.r.t. the block values for Y = A X.  Padding
+ * positions are set to zero.  Fixed reduction tree over v.
+ */
+int smax_hip_bcrs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_bcrs A,
+                        int64_t k, const void *G, const void *X, void *dvals);
+
 /*
  * SELL-C-sigma layout of A^T built once from A's device CRS pattern, with a
  * stable counting sort by column (deterministic, no atomics).  The result's
@@ -312,11 +356,16 @@ typedef struct {
     int64_t max_iters;      /* total (inner) iteration cap                   */
     int64_t check_every;    /* iterations enqueued between host checks; the
                                device stops itself on convergence (>= 1)     */
-    double inner_rel_tol;   /* IR-CG: inner solve residual reduction         */
-    int64_t max_inner;      /* IR-CG: iteration cap per inner solve          */
+    double inner_rel_tol;   /* IR-CG: residual reduction of the low-precision
+                               iteration between two FP64 refinement steps  */
+    int64_t max_inner;      /* IR-CG: iteration cap per refinement step      */
     int32_t inner_value_type; /* IR-CG: SMAX_HIP_F32 or SMAX_HIP_BF16        */
     int32_t keep_diag_f32;  /* IR-CG, BF16 storage: apply the diagonal from an
                                FP32 side array instead of the BF16 values     */
+    int32_t restart_inner;  /* IR-CG: 0 = keep the low-precision Krylov space
+                               across refinement steps (reliable updates),
+                               1 = restart CG on each correction equation
+                               A d = r (classical iterative refinement)     */
 } smax_hip_cg_params;
 
 typedef struct {
@@ -340,10 +389,16 @@ int smax_hip_cg_f64(smax_hip_ctx ctx, smax_hip_scs A, const double *b,
                     smax_hip_cg_report *report);
 
 /*
- * Mixed-precision iterative-refinement CG: the residual r = b - A x, the
- * update x += d and all outer norms and dot products in FP64 on the FP64
- * matrix; each correction d solves A d = r approximately with CG on a copy of
- * A's values stored in FP32 or BF16 (FP32 vectors and accumulation).
+ * Mixed-precision iterative-refinement CG.  The residual r = b - A x, the
+ * solution update and all refinement-loop norms and dot products are FP64 on
+ * the FP64 matrix; the iteration between refinement steps runs CG on a copy
+ * of A's values stored in FP32 or BF16 (FP32 vectors and accumulation).
+ * Default (restart_inner = 0): the low-precision CG keeps its search
+ * direction across refinement steps, and a refinement step (x += correction
+ * in FP64, r = b - A x in FP64, low-precision residual replaced by it) runs
+ * whenever the low-precision residual has dropped by inner_rel_tol since the
+ * last one ("reliable updates").  restart_inner = 1: classical refinement,
+ * a fresh CG solve of A d = r to inner_rel_tol per step.
  */
 int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b,
                   double *x, const smax_hip_cg_params *params,

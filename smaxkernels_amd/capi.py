@@ -44,21 +44,21 @@ class ScsInfo(C.Structure):
                                    "n_cols", "n_chunks", "n_elements", "nnz",
                                    "val_bytes")] + [
         ("chunk_ptr", vp), ("chunk_lengths", vp), ("col", vp), ("val", vp),
-        ("perm", vp), ("slot_row", vp), ("src_index", vp)]
+        ("perm", vp), ("slot_row", vp), ("src_index", vp), ("slot_len", vp)]
 
 
 class BcrsInfo(C.Structure):
     _fields_ = [(n, i64) for n in ("n_block_rows", "n_block_cols", "n_blocks",
                                    "b_h", "b_w", "h_pad", "w_pad", "val_bytes",
                                    "block_col_major")] + [
-        ("col", vp), ("row_ptr", vp), ("val", vp)]
+        ("col", vp), ("row_ptr", vp), ("val", vp), ("src_block", vp)]
 
 
 class CgParams(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_iters", i64),
                 ("check_every", i64), ("inner_rel_tol", C.c_double),
                 ("max_inner", i64), ("inner_value_type", C.c_int32),
-                ("keep_diag_f32", C.c_int32)]
+                ("keep_diag_f32", C.c_int32), ("restart_inner", C.c_int32)]
 
 
 class CgReport(C.Structure):
@@ -101,6 +101,7 @@ _SIGS = {
     "smax_hip_scs_get_info": ([vp, C.POINTER(ScsInfo)], C.c_int),
     "smax_hip_scs_destroy": ([vp, vp], C.c_int),
     "smax_hip_scs_refresh_values": ([vp, vp, vp], C.c_int),
+    "smax_hip_scs_apply": ([vp, C.c_int, vp, vp, vp, C.c_int], C.c_int),
     "smax_hip_bcrs_spmv": ([vp, C.c_int, i64, i64, i64, i64, i64, vp, vp, vp,
                             C.c_int, vp, vp], C.c_int),
     "smax_hip_bcrs_spmm": ([vp, C.c_int, i64, i64, i64, i64, i64, vp, vp, vp,
@@ -112,6 +113,9 @@ _SIGS = {
     "smax_hip_scs_transpose_from_crs": ([vp, i64, i64, i64, vp, vp, vp, i64,
                                          i64, i64, C.POINTER(vp)], C.c_int),
     "smax_hip_scs_sddmm": ([vp, C.c_int, vp, i64, vp, vp, vp], C.c_int),
+    "smax_hip_bcrs_transpose": ([vp, vp, C.POINTER(vp)], C.c_int),
+    "smax_hip_bcrs_refresh_values": ([vp, vp, vp], C.c_int),
+    "smax_hip_bcrs_sddmm": ([vp, C.c_int, vp, i64, vp, vp, vp], C.c_int),
     "smax_hip_cg_f64": ([vp, vp, vp, vp, C.POINTER(CgParams),
                          C.POINTER(CgReport)], C.c_int),
     "smax_hip_ircg": ([vp, vp, vp, vp, C.POINTER(CgParams),
@@ -373,6 +377,7 @@ class Scs:
             "perm": d(i.perm, i.n_rows, np.int32),
             "slot_row": d(i.slot_row, i.n_rows_padded, np.int32),
             "src_index": d(i.src_index, i.n_elements, np.int32),
+            "slot_len": d(i.slot_len, i.n_rows_padded, np.int32),
         }
 
     def spmv(self, vtype, x, y, unpermuted=False, val=None) -> None:
@@ -386,6 +391,11 @@ class Scs:
             c._check(c.lib.smax_hip_scs_spmv(
                 c.handle, vtype, i.C, i.n_chunks, i.chunk_ptr, i.chunk_lengths,
                 i.col, v, _ptr(x), _ptr(y)))
+
+    def apply(self, vtype, x, y, unpermuted=False) -> None:
+        c = self.ctx
+        c._check(c.lib.smax_hip_scs_apply(c.handle, vtype, self.handle, _ptr(x),
+                                          _ptr(y), int(unpermuted)))
 
     def spmm(self, vtype, k, X, Y, val=None) -> None:
         i, c = self.info, self.ctx
@@ -425,6 +435,34 @@ class Bcrs:
             "row_ptr": d(i.row_ptr, i.n_block_rows + 1, np.int32),
             "val": d(i.val, i.n_blocks * i.h_pad * i.w_pad, dtype),
         }
+
+    def spmm(self, vtype, k, X, Y) -> None:
+        i, c = self.info, self.ctx
+        c._check(c.lib.smax_hip_bcrs_spmm(
+            c.handle, vtype, i.n_block_rows, i.b_h, i.b_w, i.h_pad, i.w_pad, i.col,
+            i.row_ptr, i.val, int(i.block_col_major), k, _ptr(X), _ptr(Y)))
+
+    def spmv(self, vtype, x, y) -> None:
+        i, c = self.info, self.ctx
+        c._check(c.lib.smax_hip_bcrs_spmv(
+            c.handle, vtype, i.n_block_rows, i.b_h, i.b_w, i.h_pad, i.w_pad, i.col,
+            i.row_ptr, i.val, int(i.block_col_major), _ptr(x), _ptr(y)))
+
+    def transpose(self) -> "Bcrs":
+        h = vp()
+        self.ctx._check(self.ctx.lib.smax_hip_bcrs_transpose(
+            self.ctx.handle, self.handle, C.byref(h)))
+        return Bcrs(self.ctx, h.value)
+
+    def refresh_values(self, src_val) -> None:
+        c = self.ctx
+        c._check(c.lib.smax_hip_bcrs_refresh_values(c.handle, self.handle,
+                                                    _ptr(src_val)))
+
+    def sddmm(self, vtype, k, G, X, dvals) -> None:
+        c = self.ctx
+        c._check(c.lib.smax_hip_bcrs_sddmm(c.handle, vtype, self.handle, k,
+                                           _ptr(G), _ptr(X), _ptr(dvals)))
 
     def destroy(self) -> None:
         if self.handle:

@@ -17,120 +17,10 @@
 
 #include "common.hpp"
 #include "convert_internal.hpp"
+#include "radix.hpp"
 #include "scan.hpp"
 
 namespace smaxhip {
-
-constexpr int kRThreads = 256;
-constexpr int kRItems = 16;
-constexpr int kRTile = kRThreads * kRItems;
-constexpr int kRBuckets = 16;
-
-__global__ __launch_bounds__(kRThreads) void radix_hist(
-    int64_t n, int shift, const uint32_t *__restrict__ keys,
-    int32_t *__restrict__ hist, int64_t n_tiles) {
-    __shared__ int32_t c[kRBuckets];
-    if (threadIdx.x < kRBuckets) c[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kRTile +
-                         static_cast<int64_t>(threadIdx.x) * kRItems;
-    for (int k = 0; k < kRItems; ++k) {
-        const int64_t i = base + k;
-        if (i < n) atomicAdd(&c[(keys[i] >> shift) & (kRBuckets - 1)], 1);
-    }
-    __syncthreads();
-    if (threadIdx.x < kRBuckets)
-        hist[threadIdx.x * n_tiles + blockIdx.x] = c[threadIdx.x];
-}
-
-// incl: inclusive scan of hist (digit-major), so (digit d, tile t) starts at
-// incl[d*n_tiles+t] - hist[d*n_tiles+t].
-__global__ __launch_bounds__(kRThreads) void radix_scatter(
-    int64_t n, int shift, const uint32_t *__restrict__ keys,
-    const int32_t *__restrict__ vals, const int32_t *__restrict__ incl,
-    const int32_t *__restrict__ hist, int64_t n_tiles,
-    uint32_t *__restrict__ okeys, int32_t *__restrict__ ovals) {
-    __shared__ int32_t cnt[kRBuckets * kRThreads];
-    const int tid = threadIdx.x;
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kRTile +
-                         static_cast<int64_t>(tid) * kRItems;
-    uint32_t kk[kRItems];
-    int32_t vv[kRItems];
-    int32_t mine[kRBuckets];
-    for (int d = 0; d < kRBuckets; ++d) mine[d] = 0;
-    for (int k = 0; k < kRItems; ++k) {
-        const int64_t i = base + k;
-        if (i < n) {
-            kk[k] = keys[i];
-            vv[k] = vals[i];
-            mine[(kk[k] >> shift) & (kRBuckets - 1)]++;
-        }
-    }
-    for (int d = 0; d < kRBuckets; ++d) cnt[d * kRThreads + tid] = mine[d];
-    __syncthreads();
-    // flat exclusive scan of cnt (digit-major): thread t owns [16t, 16t+16)
-    int32_t loc[kRItems];
-    int32_t s = 0;
-    for (int k = 0; k < kRItems; ++k) {
-        loc[k] = s;
-        s += cnt[tid * kRItems + k];
-    }
-    int32_t total;
-    const int32_t before = block_excl_scan(s, &total);
-    __syncthreads();
-    for (int k = 0; k < kRItems; ++k) cnt[tid * kRItems + k] = before + loc[k];
-    __syncthreads();
-    int32_t run[kRBuckets];
-    for (int d = 0; d < kRBuckets; ++d)
-        run[d] = incl[d * n_tiles + blockIdx.x] - hist[d * n_tiles + blockIdx.x] +
-                 cnt[d * kRThreads + tid] - cnt[d * kRThreads];
-    for (int k = 0; k < kRItems; ++k) {
-        const int64_t i = base + k;
-        if (i < n) {
-            const int d = (kk[k] >> shift) & (kRBuckets - 1);
-            const int32_t p = run[d]++;
-            okeys[p] = kk[k];
-            ovals[p] = vv[k];
-        }
-    }
-}
-
-__global__ void iota_keys(int64_t n, const int32_t *__restrict__ col,
-                          uint32_t *__restrict__ keys, int32_t *__restrict__ vals) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    keys[i] = static_cast<uint32_t>(col[i]);
-    vals[i] = static_cast<int32_t>(i);
-}
-
-// t_row_ptr[c] = first sorted position with key >= c, for c in [0, n_cols].
-__global__ void transpose_row_ptr(int64_t n_cols, int64_t nnz,
-                                  const uint32_t *__restrict__ keys,
-                                  int32_t *__restrict__ t_row_ptr) {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c > n_cols) return;
-    int64_t a = 0, b = nnz;
-    while (a < b) {
-        const int64_t m = (a + b) >> 1;
-        if (keys[m] < static_cast<uint32_t>(c)) a = m + 1; else b = m;
-    }
-    t_row_ptr[c] = static_cast<int32_t>(a);
-}
-
-__global__ void transpose_cols(int64_t nnz, int64_t n_rows,
-                               const int32_t *__restrict__ row_ptr,
-                               const int32_t *__restrict__ src,
-                               int32_t *__restrict__ t_col) {
-    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= nnz) return;
-    const int64_t j = src[p];
-    int64_t lo = 0, hi = n_rows;
-    while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (row_ptr[mid] <= j) lo = mid; else hi = mid;
-    }
-    t_col[p] = static_cast<int32_t>(lo);
-}
 
 // ------------------------------------------------------------------ SDDMM
 template <int VT, int G>
@@ -215,74 +105,39 @@ int smax_hip_scs_transpose_from_crs(smax_hip_ctx ctx, int64_t n_rows,
                               nnz <= INT32_MAX && row_ptr &&
                               (nnz == 0 || (col && val)),
                      "smax_hip_scs_transpose_from_crs: bad input");
-    const int64_t n_tiles = ceil_div(nnz > 0 ? nnz : 1, kRTile);
-    uint32_t *k0 = nullptr, *k1 = nullptr;
-    int32_t *v0 = nullptr, *v1 = nullptr, *hist = nullptr, *tmp = nullptr;
-    int32_t *incl = nullptr;
-    int32_t *t_row_ptr = nullptr, *t_col = nullptr;
     const int64_t m = nnz > 0 ? nnz : 1;
-    int rc = dev_alloc(ctx, reinterpret_cast<void **>(&k0), 4 * m, "transpose");
-    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&k1), 4 * m, "transpose");
-    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&v0), 4 * m, "transpose");
-    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&v1), 4 * m, "transpose");
-    if (!rc)
-        rc = dev_alloc(ctx, reinterpret_cast<void **>(&hist),
-                       4 * kRBuckets * n_tiles, "transpose");
-    if (!rc)
-        rc = dev_alloc(ctx, reinterpret_cast<void **>(&incl),
-                       4 * kRBuckets * n_tiles, "transpose");
-    if (!rc)
-        rc = dev_alloc(ctx, reinterpret_cast<void **>(&tmp),
-                       4 * scan_tmp_elems(kRBuckets * n_tiles), "transpose");
+    uint32_t *keys = nullptr;
+    int32_t *vals = nullptr, *t_row_ptr = nullptr, *t_col = nullptr;
+    RadixSort sorter(ctx);
+    int rc = sorter.reserve(m);
     if (!rc)
         rc = dev_alloc(ctx, reinterpret_cast<void **>(&t_row_ptr),
                        4 * (n_cols + 1), "transpose");
     if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&t_col), 4 * m, "transpose");
     if (!rc && nnz > 0) {
         hipLaunchKernelGGL(iota_keys, grid_for(nnz, 256), dim3(256), 0,
-                           ctx->stream, nnz, col, k0, v0);
+                           ctx->stream, nnz, col, sorter.keys(), sorter.vals());
         rc = check_launch(ctx, "iota_keys");
-        int bits = 1;
-        while (bits < 32 && (1ll << bits) < n_cols) ++bits;
-        for (int shift = 0; !rc && shift < bits; shift += 4) {
-            hipLaunchKernelGGL(radix_hist, dim3(static_cast<unsigned>(n_tiles)),
-                               dim3(kRThreads), 0, ctx->stream, nnz, shift, k0,
-                               hist, n_tiles);
-            rc = check_launch(ctx, "radix_hist");
-            if (!rc)
-                rc = scan_inclusive<int32_t, false>(ctx, hist, incl,
-                                                    kRBuckets * n_tiles, tmp);
-            if (!rc) {
-                hipLaunchKernelGGL(radix_scatter,
-                                   dim3(static_cast<unsigned>(n_tiles)),
-                                   dim3(kRThreads), 0, ctx->stream, nnz, shift,
-                                   k0, v0, incl, hist, n_tiles, k1, v1);
-                rc = check_launch(ctx, "radix_scatter");
-            }
-            std::swap(k0, k1);
-            std::swap(v0, v1);
-        }
+        if (!rc) rc = sorter.sort(nnz, n_cols);
     }
+    keys = sorter.keys();
+    vals = sorter.vals();
     if (!rc) {
         hipLaunchKernelGGL(transpose_row_ptr, grid_for(n_cols + 1, 256), dim3(256),
-                           0, ctx->stream, n_cols, nnz, k0, t_row_ptr);
+                           0, ctx->stream, n_cols, nnz, keys, t_row_ptr);
         rc = check_launch(ctx, "transpose_row_ptr");
     }
     if (!rc && nnz > 0) {
         hipLaunchKernelGGL(transpose_cols, grid_for(nnz, 256), dim3(256), 0,
-                           ctx->stream, nnz, n_rows, row_ptr, v0, t_col);
+                           ctx->stream, nnz, n_rows, row_ptr, vals, t_col);
         rc = check_launch(ctx, "transpose_cols");
     }
     if (!rc)
         rc = build_scs(ctx, n_cols, n_rows, nnz, t_row_ptr, t_col, val,
-                       val_bytes, v0, C, sigma, out);
+                       val_bytes, vals, C, sigma, out);
     (void)hipStreamSynchronize(ctx->stream);
-    for (void *p : {static_cast<void *>(k0), static_cast<void *>(k1),
-                    static_cast<void *>(v0), static_cast<void *>(v1),
-                    static_cast<void *>(hist), static_cast<void *>(incl),
-                    static_cast<void *>(tmp),
-                    static_cast<void *>(t_row_ptr), static_cast<void *>(t_col)})
-        dev_free(p);
+    dev_free(t_row_ptr);
+    dev_free(t_col);
     return rc;
 }
 

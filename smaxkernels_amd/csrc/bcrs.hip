@@ -407,8 +407,204 @@ int smax_hip_bcrs_destroy(smax_hip_ctx ctx, smax_hip_bcrs bcrs) {
     dev_free(bcrs->info.row_ptr);
     dev_free(bcrs->info.col);
     dev_free(bcrs->info.val);
+    dev_free(bcrs->info.src_block);
     delete bcrs;
     return SMAX_HIP_SUCCESS;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ adjoint
+#include "radix.hpp"
+
+namespace smaxhip {
+
+template <typename V>
+__global__ void bcrs_gather_blocks(int64_t total, int64_t bs,
+                                   const int32_t *__restrict__ src_block,
+                                   const V *__restrict__ src, V *__restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int64_t p = i / bs, t = i - p * bs;
+    out[i] = src[static_cast<int64_t>(src_block[p]) * bs + t];
+}
+
+static int gather_blocks(smax_hip_ctx ctx, const smax_hip_bcrs_info &T,
+                         const void *src) {
+    const int64_t bs = T.h_pad * T.w_pad, total = T.n_blocks * bs;
+    if (total == 0) return SMAX_HIP_SUCCESS;
+    const dim3 g = grid_for(total, 256);
+    if (T.val_bytes == 8)
+        hipLaunchKernelGGL(bcrs_gather_blocks<uint64_t>, g, dim3(256), 0, ctx->stream,
+                           total, bs, T.src_block, static_cast<const uint64_t *>(src),
+                           static_cast<uint64_t *>(T.val));
+    else if (T.val_bytes == 4)
+        hipLaunchKernelGGL(bcrs_gather_blocks<uint32_t>, g, dim3(256), 0, ctx->stream,
+                           total, bs, T.src_block, static_cast<const uint32_t *>(src),
+                           static_cast<uint32_t *>(T.val));
+    else
+        hipLaunchKernelGGL(bcrs_gather_blocks<uint16_t>, g, dim3(256), 0, ctx->stream,
+                           total, bs, T.src_block, static_cast<const uint16_t *>(src),
+                           static_cast<uint16_t *>(T.val));
+    SMAX_HIP_LAUNCHED(ctx, "bcrs_gather_blocks");
+    return SMAX_HIP_SUCCESS;
+}
+
+// Group of G lanes per block position; fixed xor-butterfly over the group.
+template <int VT, int G>
+__global__ __launch_bounds__(256) void bcrs_sddmm_kernel(
+    int64_t n_positions, int64_t n_block_rows, int b_h, int b_w, int h_pad, int w_pad,
+    int col_major, int64_t k, const int32_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ col, const typename vt<VT>::vec_t *__restrict__ Gm,
+    const typename vt<VT>::vec_t *__restrict__ X,
+    typename vt<VT>::vec_t *__restrict__ dvals) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t e = t / G;
+    const int lane = static_cast<int>(t % G);
+    const int64_t bs = static_cast<int64_t>(h_pad) * w_pad;
+    const int64_t j = e / bs, off = e - j * bs;
+    const int ix = static_cast<int>(col_major ? off % h_pad : off / w_pad);
+    const int jx = static_cast<int>(col_major ? off / h_pad : off % w_pad);
+    const bool real = e < n_positions && ix < b_h && jx < b_w;
+    A acc = 0;
+    if (real) {
+        int64_t lo = 0, hi = n_block_rows;  // block row of block j
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (row_ptr[mid] <= j) lo = mid; else hi = mid;
+        }
+        const A *gr = Gm + (lo * h_pad + ix) * k;
+        const A *xr = X + (static_cast<int64_t>(col[j]) * w_pad + jx) * k;
+        for (int64_t v = lane; v < k; v += G) acc = T::fma_(gr[v], xr[v], acc);
+    }
+#pragma unroll
+    for (int o = G / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, G);
+    if (e < n_positions && lane == 0) dvals[e] = real ? acc : A(0);
+}
+
+template <int VT, int G>
+static void launch_bcrs_sddmm_g(smax_hip_ctx ctx, const smax_hip_bcrs_info &in,
+                                int64_t k, const void *Gm, const void *X, void *dv) {
+    using T = vt<VT>;
+    const int64_t n = in.n_blocks * in.h_pad * in.w_pad;
+    hipLaunchKernelGGL((bcrs_sddmm_kernel<VT, G>), grid_for(n * G, 256), dim3(256), 0,
+                       ctx->stream, n, in.n_block_rows, static_cast<int>(in.b_h),
+                       static_cast<int>(in.b_w), static_cast<int>(in.h_pad),
+                       static_cast<int>(in.w_pad), static_cast<int>(in.block_col_major),
+                       k, in.row_ptr, in.col,
+                       static_cast<const typename T::vec_t *>(Gm),
+                       static_cast<const typename T::vec_t *>(X),
+                       static_cast<typename T::vec_t *>(dv));
+}
+
+template <int VT>
+static int launch_bcrs_sddmm(smax_hip_ctx ctx, const smax_hip_bcrs_info &in,
+                             int64_t k, int g, const void *Gm, const void *X,
+                             void *dv) {
+    switch (g) {
+    case 1: launch_bcrs_sddmm_g<VT, 1>(ctx, in, k, Gm, X, dv); break;
+    case 2: launch_bcrs_sddmm_g<VT, 2>(ctx, in, k, Gm, X, dv); break;
+    case 4: launch_bcrs_sddmm_g<VT, 4>(ctx, in, k, Gm, X, dv); break;
+    case 8: launch_bcrs_sddmm_g<VT, 8>(ctx, in, k, Gm, X, dv); break;
+    case 16: launch_bcrs_sddmm_g<VT, 16>(ctx, in, k, Gm, X, dv); break;
+    case 32: launch_bcrs_sddmm_g<VT, 32>(ctx, in, k, Gm, X, dv); break;
+    default: launch_bcrs_sddmm_g<VT, 64>(ctx, in, k, Gm, X, dv); break;
+    }
+    SMAX_HIP_LAUNCHED(ctx, "bcrs_sddmm_kernel");
+    return SMAX_HIP_SUCCESS;
+}
+
+}  // namespace smaxhip
+
+extern "C" {
+
+int smax_hip_bcrs_transpose(smax_hip_ctx ctx, smax_hip_bcrs A, smax_hip_bcrs *out) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, A && out, "smax_hip_bcrs_transpose: null argument");
+    *out = nullptr;
+    const smax_hip_bcrs_info &a = A->info;
+    const int64_t nb = a.n_blocks, m = nb > 0 ? nb : 1;
+    smax_hip_bcrs_info t{};
+    t.n_block_rows = a.n_block_cols;
+    t.n_block_cols = a.n_block_rows;
+    t.n_blocks = nb;
+    t.b_h = a.b_w;
+    t.b_w = a.b_h;
+    t.h_pad = a.w_pad;
+    t.w_pad = a.h_pad;
+    t.val_bytes = a.val_bytes;
+    t.block_col_major = a.block_col_major ? 0 : 1;
+    RadixSort sorter(ctx);
+    int rc = sorter.reserve(m);
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&t.row_ptr),
+                       4 * (t.n_block_rows + 1), "bcrs transpose");
+    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&t.col), 4 * m, "bcrs transpose");
+    if (!rc)
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&t.src_block), 4 * m,
+                       "bcrs transpose");
+    if (!rc) rc = dev_alloc(ctx, &t.val, a.val_bytes * m * a.h_pad * a.w_pad, "bcrs transpose");
+    if (!rc && nb > 0) {
+        hipLaunchKernelGGL(iota_keys, grid_for(nb, 256), dim3(256), 0, ctx->stream, nb,
+                           a.col, sorter.keys(), sorter.vals());
+        rc = check_launch(ctx, "iota_keys");
+        if (!rc) rc = sorter.sort(nb, a.n_block_cols);
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(transpose_row_ptr, grid_for(t.n_block_rows + 1, 256),
+                           dim3(256), 0, ctx->stream, t.n_block_rows, nb, sorter.keys(),
+                           t.row_ptr);
+        rc = check_launch(ctx, "transpose_row_ptr");
+    }
+    if (!rc && nb > 0) {
+        hipLaunchKernelGGL(transpose_cols, grid_for(nb, 256), dim3(256), 0, ctx->stream,
+                           nb, a.n_block_rows, a.row_ptr, sorter.vals(), t.col);
+        rc = check_launch(ctx, "transpose_cols");
+        if (!rc) {
+            const hipError_t e = hipMemcpyAsync(t.src_block, sorter.vals(), 4 * nb,
+                                                hipMemcpyDeviceToDevice, ctx->stream);
+            if (e != hipSuccess) rc = hip_fail(ctx, e, "src_block copy");
+        }
+        if (!rc) rc = gather_blocks(ctx, t, a.val);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    if (rc) {
+        dev_free(t.row_ptr);
+        dev_free(t.col);
+        dev_free(t.src_block);
+        dev_free(t.val);
+        return rc;
+    }
+    smax_hip_bcrs h = new smax_hip_bcrs_s();
+    h->info = t;
+    *out = h;
+    return SMAX_HIP_SUCCESS;
+}
+
+int smax_hip_bcrs_refresh_values(smax_hip_ctx ctx, smax_hip_bcrs T,
+                                 const void *src_val) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, T && src_val && T->info.src_block,
+                     "smax_hip_bcrs_refresh_values: not a transpose or null input");
+    return gather_blocks(ctx, T->info, src_val);
+}
+
+int smax_hip_bcrs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_bcrs A, int64_t k,
+                        const void *G, const void *X, void *dvals) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    if (!valid_value_type(value_type))
+        return fail(ctx, SMAX_HIP_ERROR_UNSUPPORTED, "smax_hip_bcrs_sddmm: value type %d",
+                    value_type);
+    SMAX_HIP_REQUIRE(ctx, A && k > 0 && G && X && dvals,
+                     "smax_hip_bcrs_sddmm: bad arguments");
+    if (A->info.n_blocks == 0) return SMAX_HIP_SUCCESS;
+    int g = 1;
+    while (g < k && g < 64) g <<= 1;
+    SMAX_HIP_DISPATCH_VT(value_type,
+                         return launch_bcrs_sddmm<VT>(ctx, A->info, k, g, G, X, dvals));
+    return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 
 }  // extern "C"

@@ -244,6 +244,52 @@ __global__ __launch_bounds__(kCgThreads) void k_add_correction(
         x64[i] = __builtin_fma(nrm, static_cast<double>(d32[i]), x64[i]);
 }
 
+// x64 += double(xl), xl = 0  (refinement step of the reliable-update CG)
+__global__ __launch_bounds__(kCgThreads) void k_fold_correction(
+    int64_t n, float *__restrict__ xl, double *__restrict__ x64) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kCgThreads) {
+        x64[i] += static_cast<double>(xl[i]);
+        xl[i] = 0.0f;
+    }
+}
+
+// r32 = float(r64), partial = sum r32^2
+__global__ __launch_bounds__(kCgThreads) void k_replace_residual(
+    int64_t n, const double *__restrict__ r64, float *__restrict__ r32,
+    float *__restrict__ partial) {
+    float sq = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kCgThreads) {
+        const float v = static_cast<float>(r64[i]);
+        r32[i] = v;
+        sq = __builtin_fmaf(v, v, sq);
+    }
+    const float t = block_sum(sq);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// r32 = float(r64), p32 = r32, xl = 0, partial = sum r32^2
+__global__ __launch_bounds__(kCgThreads) void k_start_reliable(
+    int64_t n, const double *__restrict__ r64, float *__restrict__ r32,
+    float *__restrict__ p32, float *__restrict__ xl, float *__restrict__ partial) {
+    float sq = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
+         i < n; i += static_cast<int64_t>(gridDim.x) * kCgThreads) {
+        const float v = static_cast<float>(r64[i]);
+        r32[i] = v;
+        p32[i] = v;
+        xl[i] = 0.0f;
+        sq = __builtin_fmaf(v, v, sq);
+    }
+    const float t = block_sum(sq);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void k_clear_done(int64_t *ctl) {
+    if (threadIdx.x == 0) ctl[CTL_DONE] = 0;
+}
+
 __global__ void k_to_slots(int64_t npad, int64_t n_rows,
                            const int32_t *__restrict__ slot_row,
                            const double *__restrict__ v, double *__restrict__ out) {
@@ -489,6 +535,99 @@ static int prepare(smax_hip_ctx ctx, smax_hip_scs A, int lo_type, bool want_diag
     return SMAX_HIP_SUCCESS;
 }
 
+// Mixed-precision CG with reliable updates: low-precision (V values, FP32
+// vectors) CG iterations run in device-side batches until the recursive
+// residual has dropped by `delta` since the last refinement step (or below
+// the target); then the correction is folded into the FP64 solution, the
+// FP64 residual b - A x replaces the low-precision one, and the iteration
+// continues with its search direction p (beta from the replaced residual).
+template <typename V, bool DIAG>
+static int reliable_cg(smax_hip_ctx ctx, const Mat &M, const double *val64,
+                       const V *val, const float *diag, const double *bs,
+                       double *xs, double *r64, double *partial64, double *scal64,
+                       float *xl, float *r, float *p, float *q, float *partial32,
+                       float *scal32, int64_t *ctl, double bnorm,
+                       const smax_hip_cg_params *prm, int64_t batch,
+                       int64_t *iters, int64_t *steps, double *rr_out) {
+    const int64_t n = M.n;
+    const dim3 sg = grid_for(n, kCgThreads), ug = upd_grid(n);
+    const double target = prm->rel_tol * bnorm;
+    const double delta = prm->inner_rel_tol;
+    double rr64 = 0;
+    int rc = residual(ctx, M, val64, bs, xs, r64, partial64, scal64, 0, &rr64);
+    if (rc) return rc;
+    *rr_out = rr64;
+    if (bnorm == 0 || std::sqrt(rr64) <= target) return SMAX_HIP_SUCCESS;
+    hipLaunchKernelGGL(k_start_reliable, ug, dim3(kCgThreads), 0, ctx->stream, n, r64,
+                       r, p, xl, partial32);
+    hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+                       partial32, static_cast<int64_t>(ug.x), scal32 + 0, nullptr, -1.0f);
+    hipLaunchKernelGGL(k_zero_ctl, dim3(1), dim3(64), 0, ctx->stream, ctl);
+    SMAX_HIP_LAUNCHED(ctx, "reliable start");
+    double max_r = std::sqrt(rr64), best = max_r;
+    int64_t done_it = 0, stalls = 0;
+    while (done_it < prm->max_iters) {
+        const double trig = std::max(delta * max_r, 0.5 * target);
+        const float trig2 = static_cast<float>(trig * trig);
+        int64_t h[2] = {0, 0};
+        bool triggered = false;
+        while (!triggered && done_it < prm->max_iters) {
+            const int64_t nb = std::min<int64_t>(batch, prm->max_iters - done_it);
+            for (int64_t b = 0, pp = done_it & 1; b < nb; ++b, pp = 1 - pp) {
+                const int cur = static_cast<int>(pp), nxt = 1 - cur;
+                hipLaunchKernelGGL((k_spmv_dot<V, float, DIAG>), sg, dim3(kCgThreads),
+                                   0, ctx->stream, n, M.C, M.chunk_ptr, M.chunk_len,
+                                   M.col, val, diag, p, q, partial32, ctl);
+                hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0,
+                                   ctx->stream, partial32, static_cast<int64_t>(sg.x),
+                                   scal32 + 2, ctl, -1.0f);
+                hipLaunchKernelGGL((k_update_xr<float>), ug, dim3(kCgThreads), 0,
+                                   ctx->stream, n, scal32, cur, 2, xl, r, p, q,
+                                   partial32, ctl);
+                hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0,
+                                   ctx->stream, partial32, static_cast<int64_t>(ug.x),
+                                   scal32 + nxt, ctl, trig2);
+                hipLaunchKernelGGL((k_update_p<float>), ug, dim3(kCgThreads), 0,
+                                   ctx->stream, n, scal32, nxt, cur, r, p, ctl);
+            }
+            SMAX_HIP_LAUNCHED(ctx, "reliable iteration");
+            rc = read_words(ctx, ctl, h, 2);
+            if (rc) return rc;
+            done_it = h[CTL_ITERS];
+            triggered = h[CTL_DONE] != 0;
+        }
+        // refinement step in FP64
+        ++*steps;
+        const int par = static_cast<int>(done_it & 1);  // scal32[par]: newest r.r
+        hipLaunchKernelGGL(k_fold_correction, ug, dim3(kCgThreads), 0, ctx->stream, n,
+                           xl, xs);
+        rc = residual(ctx, M, val64, bs, xs, r64, partial64, scal64, 0, &rr64);
+        if (rc) return rc;
+        *rr_out = rr64;
+        const double nr = std::sqrt(rr64);
+        if (nr <= target) break;
+        if (nr < best) {
+            best = nr;
+            stalls = 0;
+        } else if (++stalls >= 3) {
+            break;  // the low-precision iteration no longer reduces the residual
+        }
+        max_r = nr;
+        hipLaunchKernelGGL(k_replace_residual, ug, dim3(kCgThreads), 0, ctx->stream, n,
+                           r64, r, partial32);
+        hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+                           partial32, static_cast<int64_t>(ug.x), scal32 + par, nullptr,
+                           -1.0f);
+        hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(64), 0, ctx->stream, ctl);
+        // the p update the triggering iteration skipped, with the new r.r
+        hipLaunchKernelGGL((k_update_p<float>), ug, dim3(kCgThreads), 0, ctx->stream, n,
+                           scal32, par, 1 - par, r, p, ctl);
+        SMAX_HIP_LAUNCHED(ctx, "refinement step");
+    }
+    *iters = done_it;
+    return SMAX_HIP_SUCCESS;
+}
+
 static int check_solver_args(smax_hip_ctx ctx, const char *fn, smax_hip_scs A,
                              const double *b, double *x,
                              const smax_hip_cg_params *prm,
@@ -615,7 +754,25 @@ int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
     const double bnorm = std::sqrt(bb);
     int64_t total = 0, outer = 0;
     double prev = HUGE_VAL;
-    for (;;) {
+    if (!prm->restart_inner) {
+        if (lo == SMAX_HIP_F32)
+            rc = reliable_cg<float, false>(ctx, M, val, A->val_f32, nullptr, bs, xs, r64,
+                                           partial64, scal64, d, r, p, q, partial32,
+                                           scal32, ctl, bnorm, prm, batch, &total,
+                                           &outer, &rr);
+        else if (diag)
+            rc = reliable_cg<uint16_t, true>(ctx, M, val, A->val_bf16, A->diag_f32, bs,
+                                             xs, r64, partial64, scal64, d, r, p, q,
+                                             partial32, scal32, ctl, bnorm, prm, batch,
+                                             &total, &outer, &rr);
+        else
+            rc = reliable_cg<uint16_t, false>(ctx, M, val, A->val_bf16, nullptr, bs, xs,
+                                              r64, partial64, scal64, d, r, p, q,
+                                              partial32, scal32, ctl, bnorm, prm, batch,
+                                              &total, &outer, &rr);
+        if (rc) return rc;
+    }
+    for (; prm->restart_inner;) {
         rc = residual(ctx, M, val, bs, xs, r64, partial64, scal64, 0, &rr);
         if (rc) return rc;
         const double rel = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;

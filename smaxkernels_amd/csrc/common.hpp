@@ -87,6 +87,10 @@ template <> struct vt<SMAX_HIP_BF16> {
     }
 };
 
+inline int64_t vt_val_bytes(int v) {
+    return v == SMAX_HIP_F64 ? 8 : v == SMAX_HIP_F32 ? 4 : 2;
+}
+
 inline bool valid_value_type(int v) {
     return v == SMAX_HIP_F64 || v == SMAX_HIP_F32 || v == SMAX_HIP_BF16;
 }

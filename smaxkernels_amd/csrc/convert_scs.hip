@@ -457,8 +457,9 @@ void scs_free_arrays(smax_hip_scs_info &in) {
     dev_free(in.perm);
     dev_free(in.slot_row);
     dev_free(in.src_index);
+    dev_free(in.slot_len);
     in.chunk_ptr = in.chunk_lengths = in.col = in.perm = in.slot_row =
-        in.src_index = nullptr;
+        in.src_index = in.slot_len = nullptr;
     in.val = nullptr;
 }
 
@@ -608,6 +609,8 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
                     sizeof(int32_t) * n_rows, "scs perm");
     rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.slot_row),
                     sizeof(int32_t) * n, "scs slot_row");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.slot_len),
+                    sizeof(int32_t) * n, "scs slot_len");
     if (rc) return SMAX_HIP_ERROR_OUT_OF_MEMORY;
 
     LAUNCH(k_chunk_len, n_chunks, n_chunks, C, cnt, info.chunk_lengths, elems);
@@ -623,6 +626,8 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
     info.n_elements = n_elements;
     LAUNCH(k_chunk_ptr, n_chunks + 1, n_chunks, incl, info.chunk_ptr);
     LAUNCH(k_perm, n, n, n_rows, idx, info.perm, info.slot_row);
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(info.slot_len, cnt, sizeof(int32_t) * n,
+                                      hipMemcpyDeviceToDevice, ctx->stream));
 
     rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.col),
                     sizeof(int32_t) * n_elements, "scs col");

@@ -61,22 +61,33 @@ slot_chain(const int32_t *__restrict__ col,
 }
 
 // C == 64: one wavefront per chunk, one lane per slot.
-template <int VT, bool UNPERM>
+//   REMAP: XCD-contiguous workgroup order (see xcd_block).
+//   LEN:   each lane stops at its row's last stored element (slot_len); a
+//          lane with padding then applies the reference's padding term
+//          fma(0, x[0], acc) once, which equals applying it any number of
+//          times (f(a) = fma(0, x0, a) satisfies f(f(a)) = f(a) for every a,
+//          signed zeros and non-finite x0 included), so results stay
+//          bit-identical to the full-chunk chain.
+template <int VT, bool UNPERM, bool REMAP, bool LEN>
 __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
     int64_t n_chunks, int64_t n_rows, const int32_t *__restrict__ chunk_ptr,
-    const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ slot_len,
+    const int32_t *__restrict__ col,
     const typename vt<VT>::val_t *__restrict__ val,
     const int32_t *__restrict__ slot_row,
     const typename vt<VT>::vec_t *__restrict__ x,
     typename vt<VT>::vec_t *__restrict__ y) {
-    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    using T = vt<VT>;
+    const int64_t blk = REMAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const int64_t chunk = blk * (kScsThreads / kWave) + (threadIdx.x / kWave);
     if (chunk >= n_chunks) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + lane;
-    const int len = chunk_len[chunk];
-    const auto acc = slot_chain<VT, kScsUnroll>(col, val, x, base, kWave, len);
     const int64_t slot = chunk * kWave + lane;
+    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + lane;
+    const int clen = chunk_len[chunk];
+    const int len = LEN ? slot_len[slot] : clen;
+    auto acc = slot_chain<VT, kScsUnroll>(col, val, x, base, kWave, len);
+    if (LEN && len < clen) acc = T::fma_(T::widen(typename T::val_t(0)), x[0], acc);
     if (UNPERM) {
         const int32_t r = slot_row[slot];
         if (r < n_rows) y[r] = acc;
@@ -156,42 +167,74 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmm_kernel(
     Y[static_cast<int64_t>(r) * k + v] = acc;
 }
 
+template <int VT, bool UNPERM, bool REMAP, bool LEN>
+static void launch_w64(smax_hip_ctx ctx, int64_t n_chunks, int64_t n_rows,
+                       const int32_t *chunk_ptr, const int32_t *chunk_len,
+                       const int32_t *slot_len, const int32_t *col, const void *val,
+                       const int32_t *slot_row, const void *x, void *y) {
+    using T = vt<VT>;
+    hipLaunchKernelGGL((scs_spmv_w64<VT, UNPERM, REMAP, LEN>),
+                       grid_for(n_chunks, kScsThreads / kWave), dim3(kScsThreads), 0,
+                       ctx->stream, n_chunks, n_rows, chunk_ptr, chunk_len, slot_len,
+                       col, static_cast<const typename T::val_t *>(val), slot_row,
+                       static_cast<const typename T::vec_t *>(x),
+                       static_cast<typename T::vec_t *>(y));
+}
+
+template <int VT, bool UNPERM>
+static void launch_w64_sched(smax_hip_ctx ctx, bool remap, bool len, int64_t n_chunks,
+                             int64_t n_rows, const int32_t *chunk_ptr,
+                             const int32_t *chunk_len, const int32_t *slot_len,
+                             const int32_t *col, const void *val,
+                             const int32_t *slot_row, const void *x, void *y) {
+    if (remap && len)
+        launch_w64<VT, UNPERM, true, true>(ctx, n_chunks, n_rows, chunk_ptr, chunk_len,
+                                          slot_len, col, val, slot_row, x, y);
+    else if (remap)
+        launch_w64<VT, UNPERM, true, false>(ctx, n_chunks, n_rows, chunk_ptr, chunk_len,
+                                           slot_len, col, val, slot_row, x, y);
+    else if (len)
+        launch_w64<VT, UNPERM, false, true>(ctx, n_chunks, n_rows, chunk_ptr, chunk_len,
+                                           slot_len, col, val, slot_row, x, y);
+    else
+        launch_w64<VT, UNPERM, false, false>(ctx, n_chunks, n_rows, chunk_ptr, chunk_len,
+                                            slot_len, col, val, slot_row, x, y);
+}
+
+// remap: XCD-contiguous order (right when sigma-windows keep rows local);
+// slot_len: optional per-slot lengths (nullptr: full chunk chains).
 template <int VT>
 int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
                     int64_t n_rows, const int32_t *chunk_ptr,
                     const int32_t *chunk_len, const int32_t *col,
                     const void *val, const int32_t *slot_row, const void *x,
-                    void *y, bool unperm) {
+                    void *y, bool unperm, bool remap = true,
+                    const int32_t *slot_len = nullptr) {
     using T = vt<VT>;
+    if (C == kWave) {
+        const bool len = slot_len != nullptr;
+        if (unperm)
+            launch_w64_sched<VT, true>(ctx, remap, len, n_chunks, n_rows, chunk_ptr,
+                                       chunk_len, slot_len, col, val, slot_row, x, y);
+        else
+            launch_w64_sched<VT, false>(ctx, remap, len, n_chunks, n_rows, chunk_ptr,
+                                        chunk_len, slot_len, col, val, slot_row, x, y);
+        SMAX_HIP_LAUNCHED(ctx, "scs_spmv_w64");
+        return SMAX_HIP_SUCCESS;
+    }
     auto v = static_cast<const typename T::val_t *>(val);
     auto xv = static_cast<const typename T::vec_t *>(x);
     auto yv = static_cast<typename T::vec_t *>(y);
-    if (C == kWave) {
-        dim3 grid = grid_for(n_chunks, kScsThreads / kWave);
-        if (unperm)
-            hipLaunchKernelGGL((scs_spmv_w64<VT, true>), grid, dim3(kScsThreads),
-                               0, ctx->stream, n_chunks, n_rows, chunk_ptr,
-                               chunk_len, col, v, slot_row, xv, yv);
-        else
-            hipLaunchKernelGGL((scs_spmv_w64<VT, false>), grid,
-                               dim3(kScsThreads), 0, ctx->stream, n_chunks,
-                               n_rows, chunk_ptr, chunk_len, col, v, slot_row,
-                               xv, yv);
-        SMAX_HIP_LAUNCHED(ctx, "scs_spmv_w64");
-    } else {
-        dim3 grid = grid_for(n_chunks * C, kScsThreads);
-        if (unperm)
-            hipLaunchKernelGGL((scs_spmv_generic<VT, true>), grid,
-                               dim3(kScsThreads), 0, ctx->stream, C, n_chunks,
-                               n_rows, chunk_ptr, chunk_len, col, v, slot_row,
-                               xv, yv);
-        else
-            hipLaunchKernelGGL((scs_spmv_generic<VT, false>), grid,
-                               dim3(kScsThreads), 0, ctx->stream, C, n_chunks,
-                               n_rows, chunk_ptr, chunk_len, col, v, slot_row,
-                               xv, yv);
-        SMAX_HIP_LAUNCHED(ctx, "scs_spmv_generic");
-    }
+    dim3 grid = grid_for(n_chunks * C, kScsThreads);
+    if (unperm)
+        hipLaunchKernelGGL((scs_spmv_generic<VT, true>), grid, dim3(kScsThreads), 0,
+                           ctx->stream, C, n_chunks, n_rows, chunk_ptr, chunk_len, col,
+                           v, slot_row, xv, yv);
+    else
+        hipLaunchKernelGGL((scs_spmv_generic<VT, false>), grid, dim3(kScsThreads), 0,
+                           ctx->stream, C, n_chunks, n_rows, chunk_ptr, chunk_len, col,
+                           v, slot_row, xv, yv);
+    SMAX_HIP_LAUNCHED(ctx, "scs_spmv_generic");
     return SMAX_HIP_SUCCESS;
 }
 
@@ -249,6 +292,32 @@ int smax_hip_scs_spmv_unpermuted(smax_hip_ctx ctx, int value_type, int64_t C,
                                                     chunk_ptr, chunk_lengths,
                                                     col, val, slot_row, x, y,
                                                     true));
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
+                       const void *x, void *y, int unpermuted) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, A, "smax_hip_scs_apply: null matrix");
+    const smax_hip_scs_info &in = A->info;
+    int rc = check_scs_args(ctx, "smax_hip_scs_apply", value_type, in.C, in.n_chunks,
+                            in.chunk_ptr, in.chunk_lengths, in.col, in.val, x, y);
+    if (rc) return rc;
+    if (vt_val_bytes(value_type) != in.val_bytes)
+        return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                    "smax_hip_scs_apply: value type %d does not match %lld-byte values",
+                    value_type, (long long)in.val_bytes);
+    // Windows much shorter than an XCD's share keep rows (and their x bands)
+    // local: give each XCD a contiguous range.  Near-global sorting puts the
+    // longest chunks first, where a contiguous split would leave one XCD
+    // with all of them: keep the hardware's round-robin order then.
+    const bool remap = in.sigma * 16 <= in.n_rows_padded;
+    SMAX_HIP_DISPATCH_VT(value_type,
+                         return launch_scs_spmv<VT>(ctx, in.C, in.n_chunks, in.n_rows,
+                                                    in.chunk_ptr, in.chunk_lengths,
+                                                    in.col, in.val, in.slot_row, x, y,
+                                                    unpermuted != 0, remap,
+                                                    in.slot_len));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 

@@ -39,6 +39,18 @@ def assert_bitwise(a: np.ndarray, b: np.ndarray, what: str = "") -> None:
                            f"{bad[0]}: {a[bad[0]]!r} vs {b[bad[0]]!r}")
 
 
+def assert_same_values(a: np.ndarray, b: np.ndarray, what: str = "") -> None:
+    """Equal values with equal zero signs, NaN where the other has NaN (NaN
+    payloads and signs differ between the host and the GPU's default NaN)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    na, nb = np.isnan(a), np.isnan(b)
+    bad = np.flatnonzero((na != nb) | (~na & ((a != b) | (np.signbit(a) != np.signbit(b)))))
+    assert bad.size == 0, (f"{what}: {bad.size} of {a.size} differ; first at "
+                           f"{bad[0]}: {a[bad[0]]!r} vs {b[bad[0]]!r}")
+
+
 def std_sort_windows(exe, counts: np.ndarray, sigma: int, tmpdir) -> np.ndarray:
     """Index order after libstdc++ std::sort of each sigma window."""
     tmpdir = Path(tmpdir)

@@ -137,3 +137,58 @@ def test_bcrs_conversion_rejects_what_the_reference_rejects(ctx):
     with pytest.raises(capi.SmaxHipError) as e:
         gpu_bcrs(ctx, 4, 4, rp, col, np.ones(2), 2, 2, 2, 2, True)
     assert "Blocked Column" in str(e.value)
+
+
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("cm", [True, False])
+def test_bcrs_adjoint_transpose_spmm_sddmm(ctx, vt, cm):
+    """C5 at reduced size: A^T through the transposed block layout equals the
+    reference BCRS conversion of the explicit A^T (bytes), its SpMM equals the
+    reference CRS SpMM of A^T bit for bit, and the block SDDMM matches FP64."""
+    A = matgen.block7(8, 4)  # 2048 rows, dense 4x4 blocks
+    val = vals_for(vt, A.val)
+    b = gpu_bcrs(ctx, A.n_rows, A.n_cols, A.row_ptr, A.col, val, 4, 4, 4, 4, cm)
+    T = b.transpose()
+    trp, tcol, tval = oracle.crs_transpose(A.n_rows, A.n_cols, A.row_ptr, A.col, val)
+    want = oracle.crs_to_bcrs(A.n_cols, A.n_rows, trp, tcol, tval, 4, 4, 4, 4, not cm)
+    got = T.arrays(val.dtype)
+    assert T.info.block_col_major == (0 if cm else 1)
+    for f in ("col", "row_ptr"):
+        assert_bitwise(got[f], want[f], f)
+    assert_bitwise(got["val"].view(np.uint8), want["val"].view(np.uint8), "val")
+    xt = np.float64 if vt == capi.F64 else np.float32
+    for k in (1, 8, 64):
+        G = matgen.uniform_f64(11, A.n_rows * k).astype(xt)
+        dG, dZ = ctx.upload(G), ctx.zeros(A.n_cols * k, xt)
+        T.spmm(vt, k, dG, dZ)
+        assert_bitwise(ctx.download(dZ), oracle.crs_spmm(trp, tcol, tval, k, True, G),
+                       f"A^T G k={k}")
+    # refresh after a value change keeps A^T current
+    new = vals_for(vt, matgen.uniform_f64(123, A.nnz))
+    b2 = gpu_bcrs(ctx, A.n_rows, A.n_cols, A.row_ptr, A.col, new, 4, 4, 4, 4, cm)
+    T.refresh_values(b2.info.val)
+    t2 = oracle.crs_transpose(A.n_rows, A.n_cols, A.row_ptr, A.col, new)
+    w2 = oracle.crs_to_bcrs(A.n_cols, A.n_rows, *t2, 4, 4, 4, 4, not cm)
+    assert_bitwise(T.arrays(val.dtype)["val"].view(np.uint8), w2["val"].view(np.uint8))
+    # block SDDMM vs FP64 at every stored scalar position
+    k = 8
+    G = matgen.uniform_f64(12, A.n_rows * k).astype(xt)
+    X = matgen.uniform_f64(13, A.n_cols * k).astype(xt)
+    dv = ctx.zeros(b.info.n_blocks * 16, xt)
+    b.sddmm(vt, k, ctx.upload(G), ctx.upload(X), dv)
+    got = ctx.download(dv).astype(np.float64)
+    ref = oracle.sddmm_f64(A.row_ptr, A.col, k, G.astype(np.float64),
+                           X.astype(np.float64))
+    rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
+    bi = b.arrays(val.dtype)
+    br, bc = rows // 4, A.col // 4
+    blk = np.array([bi["row_ptr"][r] + np.searchsorted(
+        bi["col"][bi["row_ptr"][r]:bi["row_ptr"][r + 1]], c) for r, c in zip(br, bc)])
+    off = (A.col % 4) * 4 + rows % 4 if cm else (rows % 4) * 4 + A.col % 4
+    mine = got[blk * 16 + off]
+    scale = np.abs(G.astype(np.float64).reshape(-1, k)[rows] *
+                   X.astype(np.float64).reshape(-1, k)[A.col]).sum(axis=1)
+    u = np.finfo(xt).eps / 2
+    assert (np.abs(mine - ref) <= (np.log2(k) + 1) * u * scale + 1e-300).all()
+    for h in (T, b, b2):
+        h.destroy()

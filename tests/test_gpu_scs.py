@@ -14,7 +14,7 @@ import pytest
 from oracle import oracle
 from smaxkernels_amd import capi, matgen
 from tests.helpers import (GOLDEN, almost_equal, arr, assert_bitwise,
-                           reference_kats, std_sort_windows)
+                           assert_same_values, reference_kats, std_sort_windows)
 
 pytestmark = pytest.mark.gpu
 
@@ -45,6 +45,9 @@ def check_against_oracle(s, A, C_, sigma, val=None):
     assert_bitwise(got["slot_row"][got["perm"]], np.arange(A.n_rows, dtype=np.int32))
     assert_bitwise(np.sort(got["slot_row"]),
                    np.arange(s.info.n_rows_padded, dtype=np.int32), "slot_row")
+    want_len = np.zeros(s.info.n_rows_padded, np.int32)
+    want_len[got["perm"]] = np.diff(A.row_ptr)
+    assert_bitwise(got["slot_len"], want_len, "slot_len")
     src = got["src_index"]
     real = src >= 0
     assert real.sum() == A.nnz
@@ -90,15 +93,8 @@ def test_conversion_matches_oracle_all_sigmas(ctx, C_):
 
 
 def test_conversion_ragged_and_empty_rows(ctx):
-    rng = np.random.default_rng(3)
     n = 5003
-    L = rng.integers(0, 40, n)
-    L[rng.random(n) < 0.3] = 0
-    L[::997] = 300
-    rp = np.concatenate([[0], np.cumsum(L)]).astype(np.int32)
-    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in L]
-                         ).astype(np.int32)
-    A = Csr(n, n, rp, col, rng.uniform(-1, 1, rp[-1]))
+    A = ragged(n, 3)
     for C_, sigma in ((64, 4096), (7, 50), (32, n), (128, 1)):
         s = gpu_scs(ctx, A, C_, sigma)
         check_against_oracle(s, A, C_, sigma)
@@ -185,6 +181,59 @@ def test_scs_spmv_bit_exact(ctx, vt, C_, sigma):
     assert_bitwise(ctx.download(dy2), oracle.crs_spmv(A.row_ptr, A.col, val, x),
                    "unpermuted y vs CRS")
     for d in (dx, dy, dy2):
+        d.free()
+    s.destroy()
+
+
+def ragged(n, seed, max_len=40, empty=0.3):
+    rng = np.random.default_rng(seed)
+    L = rng.integers(0, max_len, n)
+    L[rng.random(n) < empty] = 0
+    L[::997] = 300
+    rp = np.concatenate([[0], np.cumsum(L)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in L]
+                         ).astype(np.int32)
+    return Csr(n, n, rp, col, rng.uniform(-1, 1, rp[-1]))
+
+
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 64), (64, 1 << 20), (32, 256)])
+@pytest.mark.parametrize("x0", [0.5, -0.0, np.inf, np.nan])
+def test_scs_apply_matches_full_chunk_chains(ctx, vt, C_, sigma, x0):
+    """The handle-level SpMV stops each lane at its row's length; it must
+    still equal the full padded chain bit for bit, also when x[0] is -0, inf
+    or NaN (the padding terms 0*x[0] then decide the result)."""
+    A = ragged(9001, 11)
+    val, x = values_for(vt, A)
+    x = x.copy()
+    x[0] = x0
+    s = gpu_scs(ctx, A, C_, sigma, val)
+    want_s = oracle.crs_to_scs(A.row_ptr, A.col, val, C_, sigma)
+    want = oracle.scs_spmv(C_, want_s["n_chunks"], want_s["chunk_ptr"],
+                           want_s["chunk_lengths"], want_s["col"], want_s["val"], x)
+    dx = ctx.upload(x)
+    dy = ctx.zeros(s.info.n_rows_padded, x.dtype)
+    s.apply(vt, dx, dy)
+    got = ctx.download(dy)
+    assert_same_values(got, want, "apply, permuted")
+    # padding terms reach some slots: a non-finite x[0] must show there
+    assert np.isfinite(got).all() == bool(np.isfinite(x0))
+    dy2 = ctx.zeros(A.n_rows, x.dtype)
+    s.apply(vt, dx, dy2, unpermuted=True)
+    assert_same_values(ctx.download(dy2), want[want_s["perm"]], "apply, original order")
+    for d in (dx, dy, dy2):
+        d.free()
+    s.destroy()
+
+
+def test_scs_apply_rejects_mismatched_value_type(ctx):
+    A = matgen.poisson2d(16)
+    s = gpu_scs(ctx, A, 64, 1)
+    dx, dy = ctx.zeros(A.n_cols, np.float32), ctx.zeros(s.info.n_rows_padded, np.float32)
+    with pytest.raises(capi.SmaxHipError) as e:
+        s.apply(capi.F32, dx, dy)
+    assert e.value.status == capi.ERROR_INVALID_VALUE
+    for d in (dx, dy):
         d.free()
     s.destroy()
 
