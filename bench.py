@@ -10,9 +10,12 @@ GB/s: B_alg = nnz*(8+4) + (N+1)*4 + N*8 (x) + N*8 (y) per SpMV, divided by
 the average kernel time measured with HIP events on the library's stream.
 
 Also reported: every (C, sigma) variant, CRS (lane-per-row and LDS row-block
-schedules), the 512^2 5-point Laplacian (warm and cold L2/MALL), the device
-copy bandwidth used as the measured stream peak, and the oracle's OpenMP CRS
-SpMV on the host cores (bounded sample) as the CPU baseline.
+schedules), the 512^2 5-point Laplacian (warm and cold L2/MALL), FP64 CG vs
+BF16/FP32-value refined CG on the 160^3 27-point stencil, FP32 SpMM k=32 with
+its adjoint and SDDMM (1M-row Lomax band), BCRS 4x4 BF16 SpMM forward and
+adjoint (96^3 7-point block adjacency, k = 8 and 64), the device copy
+bandwidth used as the measured stream peak, and the oracle's OpenMP CRS SpMV
+on the host cores (bounded sample) as the CPU baseline.
 """
 from __future__ import annotations
 
@@ -52,6 +55,40 @@ def time_launches(ctx, fn, steps, warmup) -> float:
     for _ in range(steps):
         fn()
     return ctx.timer_stop() / steps
+
+
+def graph_ms(ctx, fn, per_graph=200, replays=10, check=None) -> float:
+    """Average ms per call of `fn` replayed from a HIP graph that holds
+    `per_graph` calls (no host launch cost between kernels).  `check`, if
+    given, is called after the eager run and after the replays and must
+    return equal results."""
+    import torch
+    s = torch.cuda.Stream()
+    ctx.set_stream(s.cuda_stream)
+    try:
+        fn()
+        torch.cuda.synchronize()
+        want = check() if check else None
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(per_graph):
+                fn()
+        with torch.cuda.stream(s):
+            g.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(s):
+            e0.record()
+            for _ in range(replays):
+                g.replay()
+            e1.record()
+        torch.cuda.synchronize()
+        if check is not None and not np.array_equal(check(), want):
+            raise RuntimeError("graph replay result differs from the eager run")
+        return e0.elapsed_time(e1) / (replays * per_graph)
+    finally:
+        ctx.set_stream(None)
 
 
 def stream_peak(ctx, steps=10) -> dict:
@@ -109,7 +146,7 @@ def cg_section(ctx) -> dict:
     out = {"n_rows": A.n_rows, "nnz": A.nnz, "format": "SELL-64-1",
            "fill": round(S.info.n_elements / A.nnz, 4)}
 
-    def solve(mixed, lo=capi.BF16, inner=1e-6, reps=3):
+    def solve(mixed, lo=capi.BF16, inner=0.1, reps=3):
         best = None
         for _ in range(reps):
             dx = ctx.zeros(A.n_rows, np.float64)
@@ -142,6 +179,73 @@ def cg_section(ctx) -> dict:
     return out
 
 
+def c4_section(ctx, steps, warmup) -> dict:
+    """FP32 SpMM with k = 32 row-major columns on the 1M-row Lomax band matrix
+    (nnz ~ 16M): forward Y = A X (CRS and SELL-64-256), adjoint A^T W through
+    the SELL layout of A^T, and the SDDMM value gradient."""
+    k = 32
+    A = matgen.lomax_band(1_000_000)
+    log(f"C4: lomax_band N={A.n_rows} nnz={A.nnz} FP32 SpMM k={k}")
+    v32 = A.val.astype(np.float32)
+    X = matgen.uniform_f32(matgen.SEED_X, A.n_cols * k)
+    W = matgen.uniform_f32(matgen.SEED_W, A.n_rows * k)
+    rp, col, val = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(v32)
+    dX, dW = ctx.upload(X), ctx.upload(W)
+    dY, dG = ctx.zeros(A.n_rows * k, np.float32), ctx.zeros(A.n_cols * k, np.float32)
+    ddv = ctx.zeros(A.nnz, np.float32)
+    S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 4, 64, 256)
+    T = ctx.scs_transpose_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 4, 64, 256)
+    ba = A.nnz * 8 + (A.n_rows + 1) * 4 + (A.n_rows + A.n_cols) * k * 4
+    runs = {
+        "forward_crs": lambda: ctx.crs_spmm(capi.F32, A.n_rows, A.n_cols, A.nnz, rp, col,
+                                            val, k, True, dX, k, dY, k),
+        "forward_sell": lambda: S.spmm(capi.F32, k, dX, dY),
+        "adjoint_sell_AT": lambda: T.spmm(capi.F32, k, dW, dG),
+        "sddmm_sell": lambda: S.sddmm(capi.F32, k, dW, dX, ddv),
+    }
+    out = {"n_rows": A.n_rows, "nnz": A.nnz, "k": k, "B_alg": ba}
+    for name, fn in runs.items():
+        ms = time_launches(ctx, fn, steps, warmup)
+        out[name] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1),
+                     "GFLOPs": round(2 * A.nnz * k / ms / 1e6, 1)}
+    log(f"C4 {out}")
+    for d in (rp, col, val, dX, dW, dY, dG, ddv):
+        d.free()
+    S.destroy()
+    T.destroy()
+    return out
+
+
+def c5_section(ctx, steps, warmup) -> dict:
+    """BCRS 4x4 SpMM with BF16 values and FP32 X/Y (row-major), 7-point block
+    adjacency on 96^3 blocks: forward and adjoint (A^T through the
+    byte-preserving BCRS transpose) for k in {8, 64}."""
+    A = matgen.block7(96, 4)
+    log(f"C5: block7 96^3 4x4 N={A.n_rows} nnz={A.nnz} BF16 BCRS SpMM")
+    rp, col, val = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(matgen.to_bf16(A.val))
+    B = ctx.bcrs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 2, 4, 4, 4, 4, False)
+    for d in (rp, col, val):
+        d.free()
+    T = B.transpose()
+    nb = B.info.n_blocks
+    out = {"n_rows": A.n_rows, "n_blocks": nb, "values": nb * 16}
+    for k in (8, 64):
+        ba = nb * (16 * 2 + 4) + (B.info.n_block_rows + 1) * 4 + 2 * A.n_rows * k * 4
+        X = ctx.upload(matgen.uniform_f32(matgen.SEED_X, A.n_rows * k))
+        Y = ctx.zeros(A.n_rows * k, np.float32)
+        for name, M in (("forward", B), ("adjoint_AT", T)):
+            ms = time_launches(ctx, lambda: M.spmm(capi.BF16, k, X, Y), steps, warmup)
+            out[f"{name}_k{k}"] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1),
+                                   "GFLOPs": round(2 * nb * 16 * k / ms / 1e6, 1),
+                                   "B_alg": ba}
+        X.free()
+        Y.free()
+    log(f"C5 {out}")
+    B.destroy()
+    T.destroy()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,7 +257,7 @@ def main() -> int:
                     help="also time sigma in {1, 64} (17x padded, 13 GB each)")
     ap.add_argument("--only", default="",
                     help="comma list of sections to run: "
-                         "stream,sell,crs,c1,cg,cpu")
+                         "stream,sell,crs,c1,cg,c4,c5,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -182,7 +286,7 @@ def main() -> int:
     best = None
     sell = {}
     if want("sell"):
-        sigmas = [4096, A.n_rows] + ([1, 64] if args.full_sweep else [])
+        sigmas = [4096, 65536, A.n_rows] + ([1, 64] if args.full_sweep else [])
         for C_ in (64, 32):
             for sigma in sigmas:
                 t0 = time.perf_counter()
@@ -190,7 +294,7 @@ def main() -> int:
                                      C_, sigma)
                 conv_s = time.perf_counter() - t0
                 yy = ctx.zeros(s.info.n_rows_padded, np.float64)
-                ms = time_launches(ctx, lambda: s.spmv(capi.F64, dx, yy),
+                ms = time_launches(ctx, lambda: s.apply(capi.F64, dx, yy),
                                    args.steps, args.warmup)
                 gbs = balg / ms / 1e6
                 fmt = (s.info.n_elements * 12 + (s.info.n_chunks + 1) * 4 +
@@ -232,19 +336,25 @@ def main() -> int:
                                 ctx.upload(P.val), ctx.upload(px))
         ps = ctx.scs_from_crs(P.n_rows, P.n_cols, P.nnz, prp, pcol, pval, 8, 64, 1)
         pdy = ctx.zeros(ps.info.n_rows_padded, np.float64)
-        warm = time_launches(ctx, lambda: ps.spmv(capi.F64, pdx, pdy), 2000, 100)
+        warm_py = time_launches(ctx, lambda: ps.apply(capi.F64, pdx, pdy), 2000, 100)
+        warm = graph_ms(ctx, lambda: ps.apply(capi.F64, pdx, pdy),
+                        check=lambda: ctx.download(pdy))
         scrub_n = 1 << 27  # 1 GiB read + 1 GiB written between launches
         s1, s2 = ctx.zeros(scrub_n, np.float64), ctx.zeros(scrub_n, np.float64)
         cold = []
         for _ in range(20):
             ctx.stream_copy(s1, s2, scrub_n)
             ctx.timer_start()
-            ps.spmv(capi.F64, pdx, pdy)
+            ps.apply(capi.F64, pdx, pdy)
             cold.append(ctx.timer_stop())
         cold_ms = float(np.median(cold))
         extra["c1_poisson512_f64_sell64_1"] = {
             "B_alg": pb, "warm_ms": round(warm, 5),
             "warm_GBs": round(pb / warm / 1e6, 1),
+            "warm_method": "HIP graph of 200 back-to-back SpMVs, 10 replays",
+            "warm_frac_of_measured_stream": (round(pb / warm / 1e6 / peak_gbs, 3)
+                                             if peak_gbs else None),
+            "warm_host_launch_ms": round(warm_py, 5),
             "cold_ms": round(cold_ms, 5), "cold_GBs": round(pb / cold_ms / 1e6, 1),
             "cold_method": "1 GiB copy between launches, median of 20 event-timed"}
         for d in (s1, s2, pdy, prp, pcol, pval, pdx):
@@ -254,6 +364,10 @@ def main() -> int:
 
     if want("cg"):
         extra["c3_cg"] = cg_section(ctx)
+    if want("c4"):
+        extra["c4_spmm_adjoint_f32"] = c4_section(ctx, 20, 3)
+    if want("c5"):
+        extra["c5_bcrs_spmm_bf16"] = c5_section(ctx, 20, 3)
 
     base = None
     if want("cpu"):

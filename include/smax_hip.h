@@ -397,8 +397,11 @@ int smax_hip_cg_f64(smax_hip_ctx ctx, smax_hip_scs A, const double *b,
  * direction across refinement steps, and a refinement step (x += correction
  * in FP64, r = b - A x in FP64, low-precision residual replaced by it) runs
  * whenever the low-precision residual has dropped by inner_rel_tol since the
- * last one ("reliable updates").  restart_inner = 1: classical refinement,
- * a fresh CG solve of A d = r to inner_rel_tol per step.
+ * last one ("reliable updates").  A refinement step whose FP64 residual is
+ * no smaller than the previous one restarts the low-precision CG from it; a
+ * second such step in a row switches the rest of the solve to classical
+ * refinement.  restart_inner = 1: classical refinement from the start, a
+ * fresh CG solve of A d = r to inner_rel_tol per step.
  */
 int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b,
                   double *x, const smax_hip_cg_params *params,

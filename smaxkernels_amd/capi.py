@@ -221,6 +221,11 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
+    def set_stream(self, stream) -> None:
+        """Run subsequent work on `stream` (a hipStream_t as int); None restores
+        the context's own stream."""
+        self._check(self.lib.smax_hip_set_stream(self.handle, stream))
+
     def sync(self) -> None:
         self._check(self.lib.smax_hip_sync(self.handle))
 

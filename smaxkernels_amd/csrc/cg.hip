@@ -34,7 +34,8 @@ constexpr int kCgThreads = 256;
 constexpr int64_t kUpdBlocks = 4096;  // fixed grid of the vector kernels
 
 // Device control words of one CG loop.
-enum : int { CTL_DONE = 0, CTL_ITERS = 1 };
+// CTL_BREAK: the last stop came from a p.q <= 0 breakdown.
+enum : int { CTL_DONE = 0, CTL_ITERS = 1, CTL_BREAK = 2 };
 
 template <typename W>
 __device__ __forceinline__ W block_sum(W v) {
@@ -165,6 +166,26 @@ __global__ __launch_bounds__(kCgThreads) void k_reduce(const W *__restrict__ par
     }
 }
 
+// *out = p.q (as k_reduce), and the loop stops before alpha is used when p.q
+// is not positive (breakdown: the operator is not positive definite in the
+// working precision, or a NaN).
+template <typename W>
+__global__ __launch_bounds__(kCgThreads) void k_reduce_pq(const W *__restrict__ partial,
+                                                          int64_t m, W *__restrict__ out,
+                                                          int64_t *__restrict__ ctl) {
+    if (ctl[CTL_DONE]) return;
+    W s = 0;
+    for (int64_t i = threadIdx.x; i < m; i += kCgThreads) s += partial[i];
+    const W t = block_sum(s);
+    if (threadIdx.x == 0) {
+        *out = t;
+        if (!(t > W(0))) {
+            ctl[CTL_DONE] = 1;
+            ctl[CTL_BREAK] = 1;
+        }
+    }
+}
+
 // x += alpha p, r -= alpha q, partial = sum r.r; alpha = scal[rr] / scal[pq]
 template <typename W>
 __global__ __launch_bounds__(kCgThreads) void k_update_xr(
@@ -287,7 +308,10 @@ __global__ __launch_bounds__(kCgThreads) void k_start_reliable(
 }
 
 __global__ void k_clear_done(int64_t *ctl) {
-    if (threadIdx.x == 0) ctl[CTL_DONE] = 0;
+    if (threadIdx.x == 0) {
+        ctl[CTL_DONE] = 0;
+        ctl[CTL_BREAK] = 0;
+    }
 }
 
 __global__ void k_to_slots(int64_t npad, int64_t n_rows,
@@ -429,9 +453,9 @@ static int cg_loop(smax_hip_ctx ctx, const Mat &A, const V *val, const float *di
             hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG>), sg, dim3(kCgThreads), 0,
                                ctx->stream, n, A.C, A.chunk_ptr, A.chunk_len, A.col,
                                val, diag, p, q, partial, ctl);
-            hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kCgThreads), 0,
+            hipLaunchKernelGGL((k_reduce_pq<W>), dim3(1), dim3(kCgThreads), 0,
                                ctx->stream, partial, static_cast<int64_t>(sg.x),
-                               scal + 2, ctl, W(-1));
+                               scal + 2, ctl);
             hipLaunchKernelGGL((k_update_xr<W>), ug, dim3(kCgThreads), 0, ctx->stream,
                                n, scal, cur, 2, x, r, p, q, partial, ctl);
             hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kCgThreads), 0,
@@ -541,14 +565,22 @@ static int prepare(smax_hip_ctx ctx, smax_hip_scs A, int lo_type, bool want_diag
 // the target); then the correction is folded into the FP64 solution, the
 // FP64 residual b - A x replaces the low-precision one, and the iteration
 // continues with its search direction p (beta from the replaced residual).
+//
+// Safeguards, for low-precision operators far from A: a cycle also ends
+// after max_inner iterations or at a p.q <= 0 breakdown, and then restarts
+// from the FP64 residual (p = r).  A refinement step that does not lower the
+// FP64 residual is undone (x restored from `xbak`) and restarts the
+// iteration; after two such steps in a row `*gave_up` hands the rest of the
+// solve to restarted (defect-correction) refinement.
 template <typename V, bool DIAG>
 static int reliable_cg(smax_hip_ctx ctx, const Mat &M, const double *val64,
                        const V *val, const float *diag, const double *bs,
-                       double *xs, double *r64, double *partial64, double *scal64,
-                       float *xl, float *r, float *p, float *q, float *partial32,
-                       float *scal32, int64_t *ctl, double bnorm,
-                       const smax_hip_cg_params *prm, int64_t batch,
-                       int64_t *iters, int64_t *steps, double *rr_out) {
+                       double *xs, double *xbak, double *r64, double *partial64,
+                       double *scal64, float *xl, float *r, float *p, float *q,
+                       float *partial32, float *scal32, int64_t *ctl, double bnorm,
+                       const smax_hip_cg_params *prm, int64_t batch, int64_t max_inner,
+                       int64_t *iters, int64_t *steps, double *rr_out,
+                       bool *gave_up) {
     const int64_t n = M.n;
     const dim3 sg = grid_for(n, kCgThreads), ug = upd_grid(n);
     const double target = prm->rel_tol * bnorm;
@@ -564,23 +596,24 @@ static int reliable_cg(smax_hip_ctx ctx, const Mat &M, const double *val64,
                        partial32, static_cast<int64_t>(ug.x), scal32 + 0, nullptr, -1.0f);
     hipLaunchKernelGGL(k_zero_ctl, dim3(1), dim3(64), 0, ctx->stream, ctl);
     SMAX_HIP_LAUNCHED(ctx, "reliable start");
-    double max_r = std::sqrt(rr64), best = max_r;
-    int64_t done_it = 0, stalls = 0;
+    double max_r = std::sqrt(rr64);
+    int64_t done_it = 0, fails = 0;
     while (done_it < prm->max_iters) {
         const double trig = std::max(delta * max_r, 0.5 * target);
         const float trig2 = static_cast<float>(trig * trig);
-        int64_t h[2] = {0, 0};
-        bool triggered = false;
-        while (!triggered && done_it < prm->max_iters) {
-            const int64_t nb = std::min<int64_t>(batch, prm->max_iters - done_it);
+        const int64_t cycle_end = std::min(prm->max_iters, done_it + max_inner);
+        int64_t h[3] = {0, 0, 0};
+        bool stopped = false;
+        while (!stopped && done_it < cycle_end) {
+            const int64_t nb = std::min<int64_t>(batch, cycle_end - done_it);
             for (int64_t b = 0, pp = done_it & 1; b < nb; ++b, pp = 1 - pp) {
                 const int cur = static_cast<int>(pp), nxt = 1 - cur;
                 hipLaunchKernelGGL((k_spmv_dot<V, float, DIAG>), sg, dim3(kCgThreads),
                                    0, ctx->stream, n, M.C, M.chunk_ptr, M.chunk_len,
                                    M.col, val, diag, p, q, partial32, ctl);
-                hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0,
+                hipLaunchKernelGGL((k_reduce_pq<float>), dim3(1), dim3(kCgThreads), 0,
                                    ctx->stream, partial32, static_cast<int64_t>(sg.x),
-                                   scal32 + 2, ctl, -1.0f);
+                                   scal32 + 2, ctl);
                 hipLaunchKernelGGL((k_update_xr<float>), ug, dim3(kCgThreads), 0,
                                    ctx->stream, n, scal32, cur, 2, xl, r, p, q,
                                    partial32, ctl);
@@ -591,37 +624,59 @@ static int reliable_cg(smax_hip_ctx ctx, const Mat &M, const double *val64,
                                    ctx->stream, n, scal32, nxt, cur, r, p, ctl);
             }
             SMAX_HIP_LAUNCHED(ctx, "reliable iteration");
-            rc = read_words(ctx, ctl, h, 2);
+            rc = read_words(ctx, ctl, h, 3);
             if (rc) return rc;
             done_it = h[CTL_ITERS];
-            triggered = h[CTL_DONE] != 0;
+            stopped = h[CTL_DONE] != 0;
         }
+        // Continue p only after a regular stop: the r.r trigger skipped the
+        // p update of its iteration, which the refinement step completes.
+        bool keep_p = stopped && h[CTL_BREAK] == 0;
         // refinement step in FP64
         ++*steps;
         const int par = static_cast<int>(done_it & 1);  // scal32[par]: newest r.r
+        SMAX_HIP_CALL(ctx, hipMemcpyAsync(xbak, xs, sizeof(double) * n,
+                                          hipMemcpyDeviceToDevice, ctx->stream));
         hipLaunchKernelGGL(k_fold_correction, ug, dim3(kCgThreads), 0, ctx->stream, n,
                            xl, xs);
         rc = residual(ctx, M, val64, bs, xs, r64, partial64, scal64, 0, &rr64);
         if (rc) return rc;
-        *rr_out = rr64;
-        const double nr = std::sqrt(rr64);
-        if (nr <= target) break;
-        if (nr < best) {
-            best = nr;
-            stalls = 0;
-        } else if (++stalls >= 3) {
-            break;  // the low-precision iteration no longer reduces the residual
+        double nr = std::sqrt(rr64);
+        if (nr <= target) {
+            *rr_out = rr64;
+            break;
         }
+        if (nr < max_r) {
+            fails = 0;
+        } else {
+            // undo the step: x and r back to the last refinement point
+            SMAX_HIP_CALL(ctx, hipMemcpyAsync(xs, xbak, sizeof(double) * n,
+                                              hipMemcpyDeviceToDevice, ctx->stream));
+            rc = residual(ctx, M, val64, bs, xs, r64, partial64, scal64, 0, &rr64);
+            if (rc) return rc;
+            nr = std::sqrt(rr64);
+            keep_p = false;
+            if (++fails >= 2) {
+                *rr_out = rr64;
+                *gave_up = true;
+                break;
+            }
+        }
+        *rr_out = rr64;
         max_r = nr;
-        hipLaunchKernelGGL(k_replace_residual, ug, dim3(kCgThreads), 0, ctx->stream, n,
-                           r64, r, partial32);
+        if (keep_p)
+            hipLaunchKernelGGL(k_replace_residual, ug, dim3(kCgThreads), 0, ctx->stream,
+                               n, r64, r, partial32);
+        else
+            hipLaunchKernelGGL(k_start_reliable, ug, dim3(kCgThreads), 0, ctx->stream,
+                               n, r64, r, p, xl, partial32);
         hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
                            partial32, static_cast<int64_t>(ug.x), scal32 + par, nullptr,
                            -1.0f);
         hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(64), 0, ctx->stream, ctl);
-        // the p update the triggering iteration skipped, with the new r.r
-        hipLaunchKernelGGL((k_update_p<float>), ug, dim3(kCgThreads), 0, ctx->stream, n,
-                           scal32, par, 1 - par, r, p, ctl);
+        if (keep_p)
+            hipLaunchKernelGGL((k_update_p<float>), ug, dim3(kCgThreads), 0, ctx->stream,
+                               n, scal32, par, 1 - par, r, p, ctl);
         SMAX_HIP_LAUNCHED(ctx, "refinement step");
     }
     *iters = done_it;
@@ -707,7 +762,7 @@ int smax_hip_cg_f64(smax_hip_ctx ctx, smax_hip_scs A, const double *b,
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     rep->iterations = total;
     rep->rel_residual = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
-    if (rep->rel_residual > prm->rel_tol)
+    if (!(rep->rel_residual <= prm->rel_tol))
         return fail(ctx, SMAX_HIP_ERROR_NOT_CONVERGED,
                     "smax_hip_cg_f64: relative residual %.3e after %lld iterations",
                     rep->rel_residual, (long long)total);
@@ -735,11 +790,12 @@ int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
     const int64_t n = M.n, batch = prm->check_every > 0 ? prm->check_every : 25;
     const int64_t max_inner = prm->max_inner > 0 ? prm->max_inner : prm->max_iters;
     Work w(ctx);
-    double *bs, *xs, *r64, *partial64, *scal64;
+    double *bs, *xs, *xbak, *r64, *partial64, *scal64;
     float *d, *r, *p, *q, *partial32, *scal32;
     int64_t *ctl;
     const int64_t np = ceil_div(n, kCgThreads) + kUpdBlocks;
-    rc = w.get(&bs, n) | w.get(&xs, n) | w.get(&r64, n) | w.get(&partial64, np) |
+    rc = w.get(&bs, n) | w.get(&xs, n) | w.get(&xbak, n) | w.get(&r64, n) |
+         w.get(&partial64, np) |
          w.get(&scal64, 8) | w.get(&d, n) | w.get(&r, n) | w.get(&p, n) |
          w.get(&q, n) | w.get(&partial32, np) | w.get(&scal32, 8) | w.get(&ctl, 4);
     if (rc) return SMAX_HIP_ERROR_OUT_OF_MEMORY;
@@ -753,31 +809,36 @@ int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
     if (rc) return rc;
     const double bnorm = std::sqrt(bb);
     int64_t total = 0, outer = 0;
-    double prev = HUGE_VAL;
+    bool fallback = false;
     if (!prm->restart_inner) {
         if (lo == SMAX_HIP_F32)
-            rc = reliable_cg<float, false>(ctx, M, val, A->val_f32, nullptr, bs, xs, r64,
-                                           partial64, scal64, d, r, p, q, partial32,
-                                           scal32, ctl, bnorm, prm, batch, &total,
-                                           &outer, &rr);
+            rc = reliable_cg<float, false>(ctx, M, val, A->val_f32, nullptr, bs, xs, xbak,
+                                           r64, partial64, scal64, d, r, p, q, partial32,
+                                           scal32, ctl, bnorm, prm, batch, max_inner,
+                                           &total, &outer, &rr, &fallback);
         else if (diag)
             rc = reliable_cg<uint16_t, true>(ctx, M, val, A->val_bf16, A->diag_f32, bs,
-                                             xs, r64, partial64, scal64, d, r, p, q,
-                                             partial32, scal32, ctl, bnorm, prm, batch,
-                                             &total, &outer, &rr);
+                                             xs, xbak, r64, partial64, scal64, d, r, p,
+                                             q, partial32, scal32, ctl, bnorm, prm,
+                                             batch, max_inner, &total, &outer, &rr,
+                                             &fallback);
         else
             rc = reliable_cg<uint16_t, false>(ctx, M, val, A->val_bf16, nullptr, bs, xs,
-                                              r64, partial64, scal64, d, r, p, q,
+                                              xbak, r64, partial64, scal64, d, r, p, q,
                                               partial32, scal32, ctl, bnorm, prm, batch,
-                                              &total, &outer, &rr);
+                                              max_inner, &total, &outer, &rr, &fallback);
         if (rc) return rc;
-    }
-    for (; prm->restart_inner;) {
+    } else {
         rc = residual(ctx, M, val, bs, xs, r64, partial64, scal64, 0, &rr);
         if (rc) return rc;
+    }
+    // Restarted refinement: each step solves A d = r (r / ||r|| in FP32) with
+    // low-precision CG, x += ||r|| d in FP64; a step that does not lower the
+    // FP64 residual is undone and ends the solve.  r64 / scal64[0] / rr hold
+    // the residual of the current x on entry to every step.
+    while (prm->restart_inner || fallback) {
         const double rel = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
-        if (rel <= prm->rel_tol || total >= prm->max_iters || !(rel < prev)) break;
-        prev = rel;
+        if (rel <= prm->rel_tol || total >= prm->max_iters) break;
         ++outer;
         hipLaunchKernelGGL(k_start_inner, upd_grid(n), dim3(kCgThreads), 0,
                            ctx->stream, n, scal64, 0, r64, r, p, d, partial32);
@@ -805,9 +866,20 @@ int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
                                                  cap, batch, &it, &rr_in);
         if (rc) return rc;
         total += it;
+        SMAX_HIP_CALL(ctx, hipMemcpyAsync(xbak, xs, sizeof(double) * n,
+                                          hipMemcpyDeviceToDevice, ctx->stream));
         hipLaunchKernelGGL(k_add_correction, upd_grid(n), dim3(kCgThreads), 0,
                            ctx->stream, n, scal64, 0, d, xs);
         SMAX_HIP_LAUNCHED(ctx, "k_add_correction");
+        double rr_new = 0;
+        rc = residual(ctx, M, val, bs, xs, r64, partial64, scal64, 0, &rr_new);
+        if (rc) return rc;
+        if (!(rr_new < rr)) {
+            SMAX_HIP_CALL(ctx, hipMemcpyAsync(xs, xbak, sizeof(double) * n,
+                                              hipMemcpyDeviceToDevice, ctx->stream));
+            break;
+        }
+        rr = rr_new;
     }
     hipLaunchKernelGGL(k_from_slots, grid_for(n, 256), dim3(256), 0, ctx->stream, n,
                        in.n_rows, in.slot_row, xs, x);
@@ -818,7 +890,7 @@ int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
     rep->iterations = total;
     rep->outer_iterations = outer;
     rep->rel_residual = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
-    if (rep->rel_residual > prm->rel_tol)
+    if (!(rep->rel_residual <= prm->rel_tol))
         return fail(ctx, SMAX_HIP_ERROR_NOT_CONVERGED,
                     "smax_hip_ircg: relative residual %.3e after %lld outer / %lld "
                     "inner iterations",

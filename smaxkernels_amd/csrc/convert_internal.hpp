@@ -11,6 +11,11 @@ struct smax_hip_scs_s {
     uint16_t *val_bf16 = nullptr; // values rounded to bfloat16
     float *diag_f32 = nullptr;    // diagonal per slot, binary32
     int bf16_split = -1;          // 1: val_bf16 holds 0 on the diagonal
+    // Execution order of the C = 64 SpMV (block slot -> chunk, -1 = none;
+    // nullptr = chunk order) and whether it assumes the XCD-contiguous block
+    // remap.  Built with the conversion.
+    int32_t *order = nullptr;
+    bool order_remap = false;
 };
 
 namespace smaxhip {
@@ -24,5 +29,8 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
               int64_t sigma, smax_hip_scs *out);
 
 void scs_free_arrays(smax_hip_scs_info &info);
+
+// Builds scs->order / scs->order_remap from the chunk lengths (synchronous).
+int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs);
 
 }  // namespace smaxhip

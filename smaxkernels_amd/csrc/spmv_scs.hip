@@ -10,6 +10,10 @@
 // The chains are unrolled: U column indices and values are loaded
 // (non-temporally, they are streamed once) before the U gathers of x and the
 // U dependent fmas, so each lane keeps 2U loads in flight.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "common.hpp"
 #include "convert_internal.hpp"
 
@@ -17,6 +21,7 @@ namespace smaxhip {
 
 constexpr int kScsThreads = 256;
 constexpr int kScsUnroll = 8;
+constexpr int kScsPipeUnroll = 16;
 
 // Workgroup order remapped so each XCD (workgroups are dealt round-robin to
 // the 8 XCDs) walks one contiguous range of chunks: neighbouring chunks, which
@@ -60,8 +65,65 @@ slot_chain(const int32_t *__restrict__ col,
     return acc;
 }
 
+// As slot_chain, software-pipelined: the next group's column indices and
+// values are requested before this group's x gathers, so a long chain waits
+// about one memory round trip per group instead of two.
+template <int VT, int U>
+__device__ __forceinline__ typename vt<VT>::vec_t
+slot_chain_pipe(const int32_t *__restrict__ col,
+                const typename vt<VT>::val_t *__restrict__ val,
+                const typename vt<VT>::vec_t *__restrict__ x, int64_t base,
+                int64_t stride, int len) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    using V = typename T::val_t;
+    A acc = 0;
+    int j = 0;
+    if (len >= U) {
+        int32_t c0[U];
+        V v0[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c0[u] = ld_stream(col + base + u * stride);
+            v0[u] = ld_stream(val + base + u * stride);
+        }
+        for (; j + 2 * U <= len; j += U) {
+            int32_t c1[U];
+            V v1[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c1[u] = ld_stream(col + base + (j + U + u) * stride);
+                v1[u] = ld_stream(val + base + (j + U + u) * stride);
+            }
+            A xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[c0[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc = T::fma_(T::widen(v0[u]), xv[u], acc);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c0[u] = c1[u];
+                v0[u] = v1[u];
+            }
+        }
+        A xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c0[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = T::fma_(T::widen(v0[u]), xv[u], acc);
+        j += U;
+    }
+    for (; j < len; ++j) {
+        const int32_t c = ld_stream(col + base + j * stride);
+        const auto v = ld_stream(val + base + j * stride);
+        acc = T::fma_(T::widen(v), x[c], acc);
+    }
+    return acc;
+}
+
 // C == 64: one wavefront per chunk, one lane per slot.
 //   REMAP: XCD-contiguous workgroup order (see xcd_block).
+//   order: optional execution order, wave position -> chunk (-1: idle).
 //   LEN:   each lane stops at its row's last stored element (slot_len); a
 //          lane with padding then applies the reference's padding term
 //          fma(0, x[0], acc) once, which equals applying it any number of
@@ -72,21 +134,22 @@ template <int VT, bool UNPERM, bool REMAP, bool LEN>
 __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
     int64_t n_chunks, int64_t n_rows, const int32_t *__restrict__ chunk_ptr,
     const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ slot_len,
-    const int32_t *__restrict__ col,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ col,
     const typename vt<VT>::val_t *__restrict__ val,
     const int32_t *__restrict__ slot_row,
     const typename vt<VT>::vec_t *__restrict__ x,
     typename vt<VT>::vec_t *__restrict__ y) {
     using T = vt<VT>;
     const int64_t blk = REMAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int64_t chunk = blk * (kScsThreads / kWave) + (threadIdx.x / kWave);
-    if (chunk >= n_chunks) return;
+    const int64_t pos = blk * (kScsThreads / kWave) + (threadIdx.x / kWave);
+    const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
+    if (chunk < 0 || chunk >= n_chunks) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t slot = chunk * kWave + lane;
     const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + lane;
     const int clen = chunk_len[chunk];
     const int len = LEN ? slot_len[slot] : clen;
-    auto acc = slot_chain<VT, kScsUnroll>(col, val, x, base, kWave, len);
+    auto acc = slot_chain_pipe<VT, kScsPipeUnroll>(col, val, x, base, kWave, len);
     if (LEN && len < clen) acc = T::fma_(T::widen(typename T::val_t(0)), x[0], acc);
     if (UNPERM) {
         const int32_t r = slot_row[slot];
@@ -167,58 +230,59 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmm_kernel(
     Y[static_cast<int64_t>(r) * k + v] = acc;
 }
 
+struct W64Args {
+    int64_t n_chunks, n_rows;
+    const int32_t *chunk_ptr, *chunk_len, *slot_len, *order, *col;
+    const void *val;
+    const int32_t *slot_row;
+    const void *x;
+    void *y;
+};
+
 template <int VT, bool UNPERM, bool REMAP, bool LEN>
-static void launch_w64(smax_hip_ctx ctx, int64_t n_chunks, int64_t n_rows,
-                       const int32_t *chunk_ptr, const int32_t *chunk_len,
-                       const int32_t *slot_len, const int32_t *col, const void *val,
-                       const int32_t *slot_row, const void *x, void *y) {
+static void launch_w64(smax_hip_ctx ctx, const W64Args &a) {
     using T = vt<VT>;
     hipLaunchKernelGGL((scs_spmv_w64<VT, UNPERM, REMAP, LEN>),
-                       grid_for(n_chunks, kScsThreads / kWave), dim3(kScsThreads), 0,
-                       ctx->stream, n_chunks, n_rows, chunk_ptr, chunk_len, slot_len,
-                       col, static_cast<const typename T::val_t *>(val), slot_row,
-                       static_cast<const typename T::vec_t *>(x),
-                       static_cast<typename T::vec_t *>(y));
+                       grid_for(a.n_chunks, kScsThreads / kWave), dim3(kScsThreads), 0,
+                       ctx->stream, a.n_chunks, a.n_rows, a.chunk_ptr, a.chunk_len,
+                       a.slot_len, a.order, a.col,
+                       static_cast<const typename T::val_t *>(a.val), a.slot_row,
+                       static_cast<const typename T::vec_t *>(a.x),
+                       static_cast<typename T::vec_t *>(a.y));
 }
 
 template <int VT, bool UNPERM>
-static void launch_w64_sched(smax_hip_ctx ctx, bool remap, bool len, int64_t n_chunks,
-                             int64_t n_rows, const int32_t *chunk_ptr,
-                             const int32_t *chunk_len, const int32_t *slot_len,
-                             const int32_t *col, const void *val,
-                             const int32_t *slot_row, const void *x, void *y) {
+static void launch_w64_sched(smax_hip_ctx ctx, bool remap, const W64Args &a) {
+    const bool len = a.slot_len != nullptr;
     if (remap && len)
-        launch_w64<VT, UNPERM, true, true>(ctx, n_chunks, n_rows, chunk_ptr, chunk_len,
-                                          slot_len, col, val, slot_row, x, y);
+        launch_w64<VT, UNPERM, true, true>(ctx, a);
     else if (remap)
-        launch_w64<VT, UNPERM, true, false>(ctx, n_chunks, n_rows, chunk_ptr, chunk_len,
-                                           slot_len, col, val, slot_row, x, y);
+        launch_w64<VT, UNPERM, true, false>(ctx, a);
     else if (len)
-        launch_w64<VT, UNPERM, false, true>(ctx, n_chunks, n_rows, chunk_ptr, chunk_len,
-                                           slot_len, col, val, slot_row, x, y);
+        launch_w64<VT, UNPERM, false, true>(ctx, a);
     else
-        launch_w64<VT, UNPERM, false, false>(ctx, n_chunks, n_rows, chunk_ptr, chunk_len,
-                                            slot_len, col, val, slot_row, x, y);
+        launch_w64<VT, UNPERM, false, false>(ctx, a);
 }
 
 // remap: XCD-contiguous order (right when sigma-windows keep rows local);
-// slot_len: optional per-slot lengths (nullptr: full chunk chains).
+// slot_len: optional per-slot lengths (nullptr: full chunk chains);
+// order: optional execution order for C = 64 (nullptr: chunk order).
 template <int VT>
 int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
                     int64_t n_rows, const int32_t *chunk_ptr,
                     const int32_t *chunk_len, const int32_t *col,
                     const void *val, const int32_t *slot_row, const void *x,
                     void *y, bool unperm, bool remap = true,
-                    const int32_t *slot_len = nullptr) {
+                    const int32_t *slot_len = nullptr,
+                    const int32_t *order = nullptr) {
     using T = vt<VT>;
     if (C == kWave) {
-        const bool len = slot_len != nullptr;
+        const W64Args a{n_chunks, n_rows, chunk_ptr, chunk_len, slot_len, order,
+                        col, val, slot_row, x, y};
         if (unperm)
-            launch_w64_sched<VT, true>(ctx, remap, len, n_chunks, n_rows, chunk_ptr,
-                                       chunk_len, slot_len, col, val, slot_row, x, y);
+            launch_w64_sched<VT, true>(ctx, remap, a);
         else
-            launch_w64_sched<VT, false>(ctx, remap, len, n_chunks, n_rows, chunk_ptr,
-                                        chunk_len, slot_len, col, val, slot_row, x, y);
+            launch_w64_sched<VT, false>(ctx, remap, a);
         SMAX_HIP_LAUNCHED(ctx, "scs_spmv_w64");
         return SMAX_HIP_SUCCESS;
     }
@@ -235,6 +299,79 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
                            ctx->stream, C, n_chunks, n_rows, chunk_ptr, chunk_len, col,
                            v, slot_row, xv, yv);
     SMAX_HIP_LAUNCHED(ctx, "scs_spmv_generic");
+    return SMAX_HIP_SUCCESS;
+}
+
+// Execution schedule of the C = 64 kernel on a converted matrix.
+//
+// Block order: windows much shorter than an XCD's share keep rows (and their
+// x bands) local, so each XCD gets a contiguous range of chunks (remap).  A
+// near-global sort puts the longest chunks first, where a contiguous split
+// would leave one XCD with all of them: the hardware's round-robin order is
+// kept then.
+//
+// Chunk order: one wave runs a chunk's 64 rows; the longest chunks (rows of
+// up to thousands of entries) then set the kernel's critical path, and in
+// chunk order the last window's long chunks start late.  Within each range
+// (each XCD's, or the whole grid) chunks of at least kLongChunk entries per
+// slot (and 8x the mean) start first, longest first, one per workgroup where
+// the range has room (each on its own CU, with its own miss queue); the rest
+// follow in chunk order.  Results do not depend on the order.
+constexpr int kLongChunk = 128;
+constexpr int kLongSpread = kScsThreads / kWave;
+
+int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs) {
+    const smax_hip_scs_info &in = scs->info;
+    scs->order_remap = in.sigma * 16 <= in.n_rows_padded;
+    const int64_t n = in.n_chunks;
+    std::vector<int32_t> cl(static_cast<size_t>(n));
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(cl.data(), in.chunk_lengths, sizeof(int32_t) * n,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    const double mean = static_cast<double>(in.n_elements) / static_cast<double>(n * kWave);
+    const int64_t thr = std::max<int64_t>(kLongChunk, static_cast<int64_t>(std::ceil(8 * mean)));
+    const int64_t n_long = std::count_if(cl.begin(), cl.end(),
+                                         [&](int32_t l) { return l >= thr; });
+    if (n_long == 0) return SMAX_HIP_SUCCESS;
+    const int64_t per_block = kScsThreads / kWave;
+    const int64_t nb = ceil_div(n, per_block);
+    std::vector<int32_t> ord(static_cast<size_t>(nb * per_block), -1);
+    // positions [a, b) hold chunks [a, min(b, n)) in the new order
+    auto fill = [&](int64_t a, int64_t b) {
+        const int64_t e = std::min(b, n);
+        if (e <= a) return;
+        std::vector<int32_t> lg, rest;
+        for (int64_t c = a; c < e; ++c) (cl[c] >= thr ? lg : rest).push_back(static_cast<int32_t>(c));
+        std::stable_sort(lg.begin(), lg.end(),
+                         [&](int32_t p, int32_t q) { return cl[p] > cl[q]; });
+        const int64_t m = static_cast<int64_t>(lg.size());
+        const int64_t step = (m - 1) * kLongSpread < e - a ? kLongSpread : 1;
+        std::vector<char> taken(static_cast<size_t>(e - a), 0);
+        for (int64_t i = 0; i < m; ++i) {
+            ord[a + i * step] = lg[i];
+            taken[i * step] = 1;
+        }
+        int64_t r = 0;
+        for (int64_t p = a; p < e; ++p)
+            if (!taken[p - a]) ord[p] = rest[r++];
+    };
+    if (scs->order_remap) {
+        constexpr int64_t kXcd = 8;
+        const int64_t q = nb / kXcd, rem = nb % kXcd;
+        for (int64_t k = 0; k < kXcd; ++k) {
+            const int64_t s = k < rem ? k * (q + 1) : rem * (q + 1) + (k - rem) * q;
+            const int64_t len = k < rem ? q + 1 : q;
+            fill(s * per_block, (s + len) * per_block);
+        }
+    } else {
+        fill(0, nb * per_block);
+    }
+    int rc = dev_alloc(ctx, reinterpret_cast<void **>(&scs->order),
+                       sizeof(int32_t) * ord.size(), "scs execution order");
+    if (rc) return rc;
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(scs->order, ord.data(), sizeof(int32_t) * ord.size(),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
     return SMAX_HIP_SUCCESS;
 }
 
@@ -307,17 +444,13 @@ int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
         return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
                     "smax_hip_scs_apply: value type %d does not match %lld-byte values",
                     value_type, (long long)in.val_bytes);
-    // Windows much shorter than an XCD's share keep rows (and their x bands)
-    // local: give each XCD a contiguous range.  Near-global sorting puts the
-    // longest chunks first, where a contiguous split would leave one XCD
-    // with all of them: keep the hardware's round-robin order then.
-    const bool remap = in.sigma * 16 <= in.n_rows_padded;
+    const bool remap = in.C == kWave ? A->order_remap : true;
     SMAX_HIP_DISPATCH_VT(value_type,
                          return launch_scs_spmv<VT>(ctx, in.C, in.n_chunks, in.n_rows,
                                                     in.chunk_ptr, in.chunk_lengths,
                                                     in.col, in.val, in.slot_row, x, y,
                                                     unpermuted != 0, remap,
-                                                    in.slot_len));
+                                                    in.slot_len, A->order));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 

@@ -17,14 +17,15 @@ def true_rel_residual(A, b, x):
 
 
 def solve(ctx, A, C_, sigma, mixed, lo=capi.BF16, diag=True, tol=1e-10,
-          inner=1e-4, max_iters=20000):
+          inner=1e-4, max_iters=20000, restart=0):
     rp, col, v = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
     S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, 8, C_, sigma)
     b = matgen.uniform_f64(matgen.SEED_B, A.n_rows)
     db, dx = ctx.upload(b), ctx.zeros(A.n_rows, np.float64)
     prm = capi.CgParams(rel_tol=tol, max_iters=max_iters, check_every=10,
                         inner_rel_tol=inner, max_inner=5000,
-                        inner_value_type=lo, keep_diag_f32=int(diag))
+                        inner_value_type=lo, keep_diag_f32=int(diag),
+                        restart_inner=restart)
     rep = ctx.cg(S, db, dx, prm, mixed)
     x = ctx.download(dx)
     S.destroy()
@@ -41,11 +42,17 @@ def test_fp64_cg_reaches_1e10(ctx, C_, sigma):
     assert 10 < rep.iterations < 500
 
 
+@pytest.mark.parametrize("restart", [0, 1])
+@pytest.mark.parametrize("inner", [1e-1, 1e-4])
 @pytest.mark.parametrize("lo,diag", [(capi.F32, False), (capi.BF16, True),
                                      (capi.BF16, False)])
-def test_ircg_reaches_1e10(ctx, lo, diag):
+def test_ircg_reaches_1e10(ctx, lo, diag, inner, restart):
+    """BF16 without the FP32 diagonal rounds 26.01 to 26: the low-precision
+    operator is then far enough from A that reliable updates must fall back
+    to restarted refinement, and still reach the target."""
     A = matgen.stencil27(20)
-    rep, b, x = solve(ctx, A, 64, 1, mixed=True, lo=lo, diag=diag)
+    rep, b, x = solve(ctx, A, 64, 1, mixed=True, lo=lo, diag=diag, inner=inner,
+                      restart=restart)
     assert rep.status == capi.SUCCESS, rep.rel_residual
     assert rep.rel_residual <= 1e-10
     assert true_rel_residual(A, b, x) <= 1.5e-10
