@@ -201,6 +201,16 @@ int smax_hip_scs_spmm(smax_hip_ctx ctx, int value_type, int64_t C,
                       const int32_t *slot_row, int64_t k, const void *X,
                       void *Y);
 
+/*
+ * Y = A X on a converted SELL handle (X, Y row-major, Y in original row
+ * order): each output is a sequential fma chain over exactly the row's
+ * stored entries in CRS order (slot_len bounds the chain, no padding term),
+ * so Y equals the reference's row-major CRS SpMM
+ * (spmm_cpu_crs_impl.hpp:50-89) bit for bit.  Chunks run long-first.
+ */
+int smax_hip_scs_apply_spmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
+                            int64_t k, const void *X, void *Y);
+
 /* ------------------------------------------------ CRS -> SELL-C-sigma (GPU) */
 typedef struct {
     int64_t C, sigma, n_rows, n_rows_padded, n_cols, n_chunks, n_elements,

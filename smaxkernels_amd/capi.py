@@ -94,6 +94,7 @@ _SIGS = {
                           C.c_int),
     "smax_hip_scs_spmv_unpermuted": ([vp, C.c_int, i64, i64, i64, vp, vp, vp,
                                       vp, vp, vp, vp], C.c_int),
+    "smax_hip_scs_apply_spmm": ([vp, C.c_int, vp, i64, vp, vp], C.c_int),
     "smax_hip_scs_spmm": ([vp, C.c_int, i64, i64, i64, vp, vp, vp, vp, vp, i64,
                            vp, vp], C.c_int),
     "smax_hip_scs_from_crs": ([vp, i64, i64, i64, vp, vp, vp, i64, i64, i64,
@@ -408,6 +409,12 @@ class Scs:
         c._check(c.lib.smax_hip_scs_spmm(
             c.handle, vtype, i.C, i.n_chunks, i.n_rows, i.chunk_ptr,
             i.chunk_lengths, i.col, v, i.slot_row, k, _ptr(X), _ptr(Y)))
+
+    def apply_spmm(self, vtype, k, X, Y) -> None:
+        """Y = A X over the stored entries only (equals the CRS SpMM)."""
+        c = self.ctx
+        c._check(c.lib.smax_hip_scs_apply_spmm(c.handle, vtype, self.handle, k,
+                                               _ptr(X), _ptr(Y)))
 
     def sddmm(self, vtype, k, G, X, dvals) -> None:
         c = self.ctx

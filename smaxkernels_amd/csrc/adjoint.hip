@@ -12,7 +12,9 @@
 // SDDMM: dvals[src_index[e]] = sum_v G[r*k+v] * X[col[e]*k+v] for every stored
 // element e of SELL row r; a group of up to 64 lanes takes one element, each
 // lane a strided part of v (fma chain), then a fixed xor-butterfly sums the
-// group: the reduction tree depends only on k.
+// group: the reduction tree depends only on k.  Workgroups take contiguous
+// element ranges, so the chunk of an element is found by stepping, not by a
+// search per element.
 #include <utility>
 
 #include "common.hpp"
@@ -23,9 +25,12 @@
 namespace smaxhip {
 
 // ------------------------------------------------------------------ SDDMM
+// Workgroup b takes elements [b*seg, (b+1)*seg) in storage order; its groups
+// step through them, tracking the chunk of the current element (one binary
+// search over chunk_ptr per thread at the start, then forward steps).
 template <int VT, int G>
 __global__ __launch_bounds__(256) void scs_sddmm_kernel(
-    int64_t n_elements, int64_t C, int64_t n_chunks, int64_t k,
+    int64_t n_elements, int64_t C, int64_t n_chunks, int64_t k, int64_t seg,
     const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ col,
     const int32_t *__restrict__ slot_row, const int32_t *__restrict__ src,
     const typename vt<VT>::vec_t *__restrict__ Gm,
@@ -33,38 +38,44 @@ __global__ __launch_bounds__(256) void scs_sddmm_kernel(
     typename vt<VT>::vec_t *__restrict__ dvals) {
     using T = vt<VT>;
     using A = typename T::vec_t;
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t e = t / G;
-    const int lane = static_cast<int>(t % G);
-    const bool valid = e < n_elements;
-    const int32_t s = valid ? src[e] : -1;
-    A acc = 0;
-    if (s >= 0) {
-        int64_t a = 0, b = n_chunks;  // chunk: chunk_ptr[a] <= e < chunk_ptr[a+1]
-        while (b - a > 1) {
-            const int64_t m = (a + b) >> 1;
-            if (chunk_ptr[m] <= e) a = m; else b = m;
-        }
-        // skip chunks of zero length that share the same start
-        while (chunk_ptr[a + 1] <= e) ++a;
-        const int64_t slot = a * C + (e - chunk_ptr[a]) % C;
-        const int64_t r = slot_row[slot];
-        const int64_t c = col[e];
-        for (int64_t v = lane; v < k; v += G)
-            acc = T::fma_(Gm[r * k + v], X[c * k + v], acc);
+    constexpr int kGroups = 256 / G;
+    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * seg;
+    const int64_t e1 = e0 + seg < n_elements ? e0 + seg : n_elements;
+    const int group = threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    // chunk a with chunk_ptr[a] <= e0 < chunk_ptr[a+1] (empty chunks skipped)
+    int64_t a = 0, b = n_chunks;
+    while (b - a > 1) {
+        const int64_t m = (a + b) >> 1;
+        if (chunk_ptr[m] <= e0) a = m; else b = m;
     }
+    int64_t next = chunk_ptr[a + 1];
+    for (int64_t e = e0 + group; e < e1; e += kGroups) {
+        while (next <= e) next = chunk_ptr[++a + 1];
+        const int32_t s = src[e];
+        A acc = 0;
+        if (s >= 0) {
+            const int64_t off = e - chunk_ptr[a];
+            const int64_t slot = a * C + (C == kWave ? (off & (kWave - 1)) : off % C);
+            const int64_t r = slot_row[slot];
+            const int64_t c = col[e];
+            for (int64_t v = lane; v < k; v += G)
+                acc = T::fma_(Gm[r * k + v], X[c * k + v], acc);
+        }
 #pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
-    if (s >= 0 && lane == 0) dvals[s] = acc;
+        for (int off = G / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
+        if (s >= 0 && lane == 0) dvals[s] = acc;
+    }
 }
 
 template <int VT, int G>
 static void launch_sddmm_g(smax_hip_ctx ctx, const smax_hip_scs_info &in,
                            int64_t k, const void *Gm, const void *X, void *dv) {
     using T = vt<VT>;
-    hipLaunchKernelGGL((scs_sddmm_kernel<VT, G>), grid_for(in.n_elements * G, 256),
+    const int64_t seg = (256 / G) * 16;
+    hipLaunchKernelGGL((scs_sddmm_kernel<VT, G>), grid_for(in.n_elements, seg),
                        dim3(256), 0, ctx->stream, in.n_elements, in.C,
-                       in.n_chunks, k, in.chunk_ptr, in.col, in.slot_row,
+                       in.n_chunks, k, seg, in.chunk_ptr, in.col, in.slot_row,
                        in.src_index, static_cast<const typename T::vec_t *>(Gm),
                        static_cast<const typename T::vec_t *>(X),
                        static_cast<typename T::vec_t *>(dv));

@@ -75,32 +75,90 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmv_lds(
 
 // Thread (r, ix, v), v fastest: the k lanes of one output row share each
 // block value (broadcast) and read one contiguous row of X per (j, jx).
-template <int VT>
+// BW > 0 fixes b_w at compile time: a block's b_w values and X entries are
+// then loaded together, and the next block's while this one's fmas run, so
+// the chain waits about one memory round trip per block.  The fma order is
+// the reference's (j, then jx) either way.
+template <int VT, int BW>
 __global__ __launch_bounds__(kBThreads) void bcrs_spmm_kernel(
-    int64_t n_block_rows, int b_h, int b_w, int h_pad, int w_pad, int col_major,
+    int64_t n_block_rows, int b_h, int b_w_rt, int h_pad, int w_pad, int col_major,
     int64_t k, const int32_t *__restrict__ col,
     const int32_t *__restrict__ row_ptr,
     const typename vt<VT>::val_t *__restrict__ val,
     const typename vt<VT>::vec_t *__restrict__ X,
     typename vt<VT>::vec_t *__restrict__ Y) {
     using T = vt<VT>;
+    using A = typename T::vec_t;
+    using V = typename T::val_t;
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
     const int64_t v = t % k, rix = t / k;
     const int64_t r = rix / b_h;
     const int ix = static_cast<int>(rix - r * b_h);
     if (r >= n_block_rows) return;
     const int64_t bs = static_cast<int64_t>(h_pad) * w_pad;
-    typename T::vec_t acc = 0;
-    for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; ++j) {
-        const typename T::val_t *blk = val + j * bs;
-        const typename T::vec_t *xr =
-            X + static_cast<int64_t>(col[j]) * w_pad * k + v;
-        for (int jx = 0; jx < b_w; ++jx) {
-            const int64_t off = col_major ? jx * h_pad + ix : ix * w_pad + jx;
-            acc = T::fma_(T::widen(blk[off]), xr[jx * k], acc);
+    // value (ix, jx) of a block sits at ix*vs_i + jx*vs_j
+    const int64_t vs_i = col_major ? 1 : w_pad, vs_j = col_major ? h_pad : 1;
+    const V *vrow = val + ix * vs_i;
+    A acc = 0;
+    int64_t j = row_ptr[r];
+    const int64_t je = row_ptr[r + 1];
+    if (BW == 0) {
+        for (; j < je; ++j) {
+            const V *blk = vrow + j * bs;
+            const A *xr = X + static_cast<int64_t>(col[j]) * w_pad * k + v;
+            for (int jx = 0; jx < b_w_rt; ++jx)
+                acc = T::fma_(T::widen(blk[jx * vs_j]), xr[jx * k], acc);
         }
+    } else if (j < je) {
+        V w0[BW > 0 ? BW : 1];
+        A x0[BW > 0 ? BW : 1];
+        {
+            const V *blk = vrow + j * bs;
+            const A *xr = X + static_cast<int64_t>(col[j]) * w_pad * k + v;
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx) {
+                w0[jx] = blk[jx * vs_j];
+                x0[jx] = xr[jx * k];
+            }
+        }
+        for (; j + 1 < je; ++j) {
+            V w1[BW > 0 ? BW : 1];
+            A x1[BW > 0 ? BW : 1];
+            const V *blk = vrow + (j + 1) * bs;
+            const A *xr = X + static_cast<int64_t>(col[j + 1]) * w_pad * k + v;
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx) {
+                w1[jx] = blk[jx * vs_j];
+                x1[jx] = xr[jx * k];
+            }
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx) acc = T::fma_(T::widen(w0[jx]), x0[jx], acc);
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx) {
+                w0[jx] = w1[jx];
+                x0[jx] = x1[jx];
+            }
+        }
+#pragma unroll
+        for (int jx = 0; jx < BW; ++jx) acc = T::fma_(T::widen(w0[jx]), x0[jx], acc);
     }
     Y[(r * h_pad + ix) * k + v] = acc;
+}
+
+template <int VT, int BW>
+static void launch_bcrs_spmm(smax_hip_ctx ctx, int64_t threads, int64_t n_block_rows,
+                             int64_t b_h, int64_t b_w, int64_t h_pad, int64_t w_pad,
+                             bool cm, int64_t k, const int32_t *col,
+                             const int32_t *row_ptr, const void *val, const void *X,
+                             void *Y) {
+    using T = vt<VT>;
+    hipLaunchKernelGGL((bcrs_spmm_kernel<VT, BW>), grid_for(threads, kBThreads),
+                       dim3(kBThreads), 0, ctx->stream, n_block_rows,
+                       static_cast<int>(b_h), static_cast<int>(b_w),
+                       static_cast<int>(h_pad), static_cast<int>(w_pad), cm ? 1 : 0, k,
+                       col, row_ptr, static_cast<const typename T::val_t *>(val),
+                       static_cast<const typename T::vec_t *>(X),
+                       static_cast<typename T::vec_t *>(Y));
 }
 
 // ------------------------------------------------------------ conversion
@@ -256,16 +314,30 @@ int smax_hip_bcrs_spmm(smax_hip_ctx ctx, int value_type,
     SMAX_HIP_REQUIRE(ctx, k > 0 && row_ptr && X && Y,
                      "smax_hip_bcrs_spmm: bad k or null pointer");
     const int64_t threads = n_block_rows * b_h * k;
+    const bool cm = block_col_major != 0;
     SMAX_HIP_DISPATCH_VT(value_type, {
-        using T = vt<VT>;
-        hipLaunchKernelGGL((bcrs_spmm_kernel<VT>), grid_for(threads, kBThreads),
-                           dim3(kBThreads), 0, ctx->stream, n_block_rows,
-                           static_cast<int>(b_h), static_cast<int>(b_w),
-                           static_cast<int>(h_pad), static_cast<int>(w_pad),
-                           block_col_major ? 1 : 0, k, col, row_ptr,
-                           static_cast<const typename T::val_t *>(val),
-                           static_cast<const typename T::vec_t *>(X),
-                           static_cast<typename T::vec_t *>(Y));
+        switch (b_w) {
+        case 1:
+            launch_bcrs_spmm<VT, 1>(ctx, threads, n_block_rows, b_h, b_w, h_pad, w_pad,
+                                    cm, k, col, row_ptr, val, X, Y);
+            break;
+        case 2:
+            launch_bcrs_spmm<VT, 2>(ctx, threads, n_block_rows, b_h, b_w, h_pad, w_pad,
+                                    cm, k, col, row_ptr, val, X, Y);
+            break;
+        case 4:
+            launch_bcrs_spmm<VT, 4>(ctx, threads, n_block_rows, b_h, b_w, h_pad, w_pad,
+                                    cm, k, col, row_ptr, val, X, Y);
+            break;
+        case 8:
+            launch_bcrs_spmm<VT, 8>(ctx, threads, n_block_rows, b_h, b_w, h_pad, w_pad,
+                                    cm, k, col, row_ptr, val, X, Y);
+            break;
+        default:
+            launch_bcrs_spmm<VT, 0>(ctx, threads, n_block_rows, b_h, b_w, h_pad, w_pad,
+                                    cm, k, col, row_ptr, val, X, Y);
+            break;
+        }
         SMAX_HIP_LAUNCHED(ctx, "bcrs_spmm_kernel");
         return SMAX_HIP_SUCCESS;
     });

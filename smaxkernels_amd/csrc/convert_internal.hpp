@@ -16,6 +16,9 @@ struct smax_hip_scs_s {
     // remap.  Built with the conversion.
     int32_t *order = nullptr;
     bool order_remap = false;
+    // Chunk order of the SpMM kernels (any C): long chunks first, longest
+    // first, then the rest in chunk order; nullptr = chunk order.
+    int32_t *order_lf = nullptr;
 };
 
 namespace smaxhip {

@@ -159,23 +159,38 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
     }
 }
 
-// Any C: one lane per slot.
-template <int VT, bool UNPERM>
+// Any C: one lane per slot, 256 / C chunks per workgroup (C <= 256) or one
+// chunk over several workgroups; chunks in the optional execution `order`;
+// LEN as in scs_spmv_w64.
+template <int VT, bool UNPERM, bool LEN>
 __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
-    int64_t C, int64_t n_chunks, int64_t n_rows,
+    int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
     const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ chunk_len,
+    const int32_t *__restrict__ slot_len, const int32_t *__restrict__ order,
     const int32_t *__restrict__ col,
     const typename vt<VT>::val_t *__restrict__ val,
     const int32_t *__restrict__ slot_row,
     const typename vt<VT>::vec_t *__restrict__ x,
     typename vt<VT>::vec_t *__restrict__ y) {
-    const int64_t slot =
-        static_cast<int64_t>(blockIdx.x) * kScsThreads + threadIdx.x;
-    if (slot >= n_chunks * C) return;
-    const int64_t chunk = slot / C, i = slot - chunk * C;
+    using T = vt<VT>;
+    int64_t pos, i;
+    if (cpb > 0) {
+        pos = static_cast<int64_t>(blockIdx.x) * cpb + threadIdx.x / C;
+        i = threadIdx.x % C;
+        if (threadIdx.x >= cpb * C) return;
+    } else {
+        pos = blockIdx.x / bpc;
+        i = (blockIdx.x % bpc) * kScsThreads + threadIdx.x;
+        if (i >= C) return;
+    }
+    if (pos >= n_chunks) return;
+    const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
+    const int64_t slot = chunk * C + i;
     const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
-    const auto acc =
-        slot_chain<VT, kScsUnroll>(col, val, x, base, C, chunk_len[chunk]);
+    const int clen = chunk_len[chunk];
+    const int len = LEN ? slot_len[slot] : clen;
+    auto acc = slot_chain_pipe<VT, kScsUnroll>(col, val, x, base, C, len);
+    if (LEN && len < clen) acc = T::fma_(T::widen(typename T::val_t(0)), x[0], acc);
     if (UNPERM) {
         const int32_t r = slot_row[slot];
         if (r < n_rows) y[r] = acc;
@@ -186,48 +201,111 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
 
 // SpMM, X and Y row-major with k columns: thread (slot, v), v fastest, so k
 // consecutive lanes share a slot's val/col (one broadcast load) and read a
-// contiguous row of X.  Y in original row order.
-template <int VT>
+// contiguous row of X.  Y in original row order.  Workgroups take whole
+// chunks (cpb chunks of C*k threads each, or one chunk in bpc parts), in the
+// optional execution `order`.  LEN: each slot stops at its row's length and
+// no padding term is applied, so Y equals the CRS SpMM bit for bit; without
+// it the chain runs over the padded chunk (padding adds 0 * X[0, v]).  The
+// chain is software-pipelined like slot_chain_pipe.
+template <int VT, bool LEN>
 __global__ __launch_bounds__(kScsThreads) void scs_spmm_kernel(
-    int64_t C, int64_t n_chunks, int64_t n_rows, int64_t k,
+    int64_t C, int64_t n_chunks, int64_t n_rows, int64_t k, int64_t cpb, int64_t bpc,
     const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ chunk_len,
-    const int32_t *__restrict__ col,
-    const typename vt<VT>::val_t *__restrict__ val,
-    const int32_t *__restrict__ slot_row,
-    const typename vt<VT>::vec_t *__restrict__ X,
+    const int32_t *__restrict__ slot_len, const int32_t *__restrict__ order,
+    const int32_t *__restrict__ col, const typename vt<VT>::val_t *__restrict__ val,
+    const int32_t *__restrict__ slot_row, const typename vt<VT>::vec_t *__restrict__ X,
     typename vt<VT>::vec_t *__restrict__ Y) {
     using T = vt<VT>;
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * kScsThreads + threadIdx.x;
-    const int64_t slot = t / k, v = t - slot * k;
-    if (slot >= n_chunks * C) return;
+    using A = typename T::vec_t;
+    using V = typename T::val_t;
+    constexpr int U = 8;
+    const int64_t ck = C * k;
+    int64_t pos, li;
+    if (cpb > 1) {
+        pos = static_cast<int64_t>(blockIdx.x) * cpb + threadIdx.x / ck;
+        li = threadIdx.x % ck;
+    } else {
+        pos = blockIdx.x / bpc;
+        li = (blockIdx.x % bpc) * kScsThreads + threadIdx.x;
+    }
+    if (pos >= n_chunks || li >= ck) return;
+    const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
+    const int64_t i = li / k, v = li - i * k;
+    const int64_t slot = chunk * C + i;
     const int32_t r = slot_row[slot];
     if (r >= n_rows) return;
-    const int64_t chunk = slot / C, i = slot - chunk * C;
     const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
-    const int len = chunk_len[chunk];
-    typename T::vec_t acc = 0;
+    const int len = LEN ? slot_len[slot] : chunk_len[chunk];
+    A acc = 0;
     int j = 0;
-    constexpr int U = 4;
-    for (; j + U <= len; j += U) {
-        int32_t c[U];
-        typename T::val_t w[U];
+    if (len >= U) {
+        int32_t c0[U];
+        V w0[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            c[u] = col[base + (j + u) * C];
-            w[u] = val[base + (j + u) * C];
+            c0[u] = col[base + u * C];
+            w0[u] = val[base + u * C];
         }
-        typename T::vec_t xv[U];
+        for (; j + 2 * U <= len; j += U) {
+            int32_t c1[U];
+            V w1[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = X[static_cast<int64_t>(c[u]) * k + v];
+            for (int u = 0; u < U; ++u) {
+                c1[u] = col[base + (j + U + u) * C];
+                w1[u] = val[base + (j + U + u) * C];
+            }
+            A xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc = T::fma_(T::widen(w[u]), xv[u], acc);
+            for (int u = 0; u < U; ++u) xv[u] = X[static_cast<int64_t>(c0[u]) * k + v];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc = T::fma_(T::widen(w0[u]), xv[u], acc);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c0[u] = c1[u];
+                w0[u] = w1[u];
+            }
+        }
+        A xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = X[static_cast<int64_t>(c0[u]) * k + v];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = T::fma_(T::widen(w0[u]), xv[u], acc);
+        j += U;
     }
     for (; j < len; ++j) {
         const int64_t o = base + static_cast<int64_t>(j) * C;
-        acc = T::fma_(T::widen(val[o]), X[static_cast<int64_t>(col[o]) * k + v],
-                      acc);
+        acc = T::fma_(T::widen(val[o]), X[static_cast<int64_t>(col[o]) * k + v], acc);
     }
     Y[static_cast<int64_t>(r) * k + v] = acc;
+}
+
+template <int VT>
+static int launch_scs_spmm(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_t n_rows,
+                           int64_t k, const int32_t *chunk_ptr, const int32_t *chunk_len,
+                           const int32_t *slot_len, const int32_t *order,
+                           const int32_t *col, const void *val, const int32_t *slot_row,
+                           const void *X, void *Y) {
+    using T = vt<VT>;
+    const int64_t ck = C * k;
+    const int64_t cpb = ck <= kScsThreads ? kScsThreads / ck : 1;
+    const int64_t bpc = ck <= kScsThreads ? 1 : ceil_div(ck, kScsThreads);
+    const int64_t blocks = cpb > 1 ? ceil_div(n_chunks, cpb) : n_chunks * bpc;
+    SMAX_HIP_REQUIRE(ctx, blocks <= INT32_MAX, "scs spmm: grid too large");
+    auto v = static_cast<const typename T::val_t *>(val);
+    auto xv = static_cast<const typename T::vec_t *>(X);
+    auto yv = static_cast<typename T::vec_t *>(Y);
+    if (slot_len)
+        hipLaunchKernelGGL((scs_spmm_kernel<VT, true>), dim3(static_cast<unsigned>(blocks)),
+                           dim3(kScsThreads), 0, ctx->stream, C, n_chunks, n_rows, k, cpb,
+                           bpc, chunk_ptr, chunk_len, slot_len, order, col, v, slot_row,
+                           xv, yv);
+    else
+        hipLaunchKernelGGL((scs_spmm_kernel<VT, false>), dim3(static_cast<unsigned>(blocks)),
+                           dim3(kScsThreads), 0, ctx->stream, C, n_chunks, n_rows, k, cpb,
+                           bpc, chunk_ptr, chunk_len, slot_len, order, col, v, slot_row,
+                           xv, yv);
+    SMAX_HIP_LAUNCHED(ctx, "scs_spmm_kernel");
+    return SMAX_HIP_SUCCESS;
 }
 
 struct W64Args {
@@ -289,15 +367,20 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
     auto v = static_cast<const typename T::val_t *>(val);
     auto xv = static_cast<const typename T::vec_t *>(x);
     auto yv = static_cast<typename T::vec_t *>(y);
-    dim3 grid = grid_for(n_chunks * C, kScsThreads);
-    if (unperm)
-        hipLaunchKernelGGL((scs_spmv_generic<VT, true>), grid, dim3(kScsThreads), 0,
-                           ctx->stream, C, n_chunks, n_rows, chunk_ptr, chunk_len, col,
-                           v, slot_row, xv, yv);
-    else
-        hipLaunchKernelGGL((scs_spmv_generic<VT, false>), grid, dim3(kScsThreads), 0,
-                           ctx->stream, C, n_chunks, n_rows, chunk_ptr, chunk_len, col,
-                           v, slot_row, xv, yv);
+    const int64_t cpb = C <= kScsThreads ? kScsThreads / C : 0;
+    const int64_t bpc = C <= kScsThreads ? 1 : ceil_div(C, kScsThreads);
+    const int64_t blocks = cpb > 0 ? ceil_div(n_chunks, cpb) : n_chunks * bpc;
+    SMAX_HIP_REQUIRE(ctx, blocks <= INT32_MAX, "scs spmv: grid too large");
+    const dim3 grid(static_cast<unsigned>(blocks));
+#define SMAX_SCS_GENERIC(U, L)                                                       \
+    hipLaunchKernelGGL((scs_spmv_generic<VT, U, L>), grid, dim3(kScsThreads), 0,    \
+                       ctx->stream, C, n_chunks, n_rows, cpb, bpc, chunk_ptr,       \
+                       chunk_len, slot_len, order, col, v, slot_row, xv, yv)
+    if (unperm && slot_len) SMAX_SCS_GENERIC(true, true);
+    else if (unperm) SMAX_SCS_GENERIC(true, false);
+    else if (slot_len) SMAX_SCS_GENERIC(false, true);
+    else SMAX_SCS_GENERIC(false, false);
+#undef SMAX_SCS_GENERIC
     SMAX_HIP_LAUNCHED(ctx, "scs_spmv_generic");
     return SMAX_HIP_SUCCESS;
 }
@@ -320,6 +403,16 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
 constexpr int kLongChunk = 128;
 constexpr int kLongSpread = kScsThreads / kWave;
 
+static int upload_order(smax_hip_ctx ctx, const std::vector<int32_t> &ord, int32_t **out) {
+    int rc = dev_alloc(ctx, reinterpret_cast<void **>(out), sizeof(int32_t) * ord.size(),
+                       "scs execution order");
+    if (rc) return rc;
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(*out, ord.data(), sizeof(int32_t) * ord.size(),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    return SMAX_HIP_SUCCESS;
+}
+
 int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs) {
     const smax_hip_scs_info &in = scs->info;
     scs->order_remap = in.sigma * 16 <= in.n_rows_padded;
@@ -328,24 +421,26 @@ int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs) {
     SMAX_HIP_CALL(ctx, hipMemcpyAsync(cl.data(), in.chunk_lengths, sizeof(int32_t) * n,
                                       hipMemcpyDeviceToHost, ctx->stream));
     SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
-    const double mean = static_cast<double>(in.n_elements) / static_cast<double>(n * kWave);
-    const int64_t thr = std::max<int64_t>(kLongChunk, static_cast<int64_t>(std::ceil(8 * mean)));
-    const int64_t n_long = std::count_if(cl.begin(), cl.end(),
-                                         [&](int32_t l) { return l >= thr; });
+    const double mean =
+        static_cast<double>(in.n_elements) / static_cast<double>(n * in.C);
+    const int64_t thr =
+        std::max<int64_t>(kLongChunk, static_cast<int64_t>(std::ceil(8 * mean)));
+    const int64_t n_long =
+        std::count_if(cl.begin(), cl.end(), [&](int32_t l) { return l >= thr; });
     if (n_long == 0) return SMAX_HIP_SUCCESS;
-    const int64_t per_block = kScsThreads / kWave;
-    const int64_t nb = ceil_div(n, per_block);
-    std::vector<int32_t> ord(static_cast<size_t>(nb * per_block), -1);
-    // positions [a, b) hold chunks [a, min(b, n)) in the new order
-    auto fill = [&](int64_t a, int64_t b) {
+    // positions [a, b) of `ord` receive chunks [a, min(b, n)): long chunks
+    // first (longest first) at every `spread`-th position when they fit,
+    // then the rest in chunk order
+    auto fill = [&](std::vector<int32_t> &ord, int64_t a, int64_t b, int64_t spread) {
         const int64_t e = std::min(b, n);
         if (e <= a) return;
         std::vector<int32_t> lg, rest;
-        for (int64_t c = a; c < e; ++c) (cl[c] >= thr ? lg : rest).push_back(static_cast<int32_t>(c));
+        for (int64_t c = a; c < e; ++c)
+            (cl[c] >= thr ? lg : rest).push_back(static_cast<int32_t>(c));
         std::stable_sort(lg.begin(), lg.end(),
                          [&](int32_t p, int32_t q) { return cl[p] > cl[q]; });
         const int64_t m = static_cast<int64_t>(lg.size());
-        const int64_t step = (m - 1) * kLongSpread < e - a ? kLongSpread : 1;
+        const int64_t step = m > 0 && (m - 1) * spread < e - a ? spread : 1;
         std::vector<char> taken(static_cast<size_t>(e - a), 0);
         for (int64_t i = 0; i < m; ++i) {
             ord[a + i * step] = lg[i];
@@ -355,24 +450,25 @@ int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs) {
         for (int64_t p = a; p < e; ++p)
             if (!taken[p - a]) ord[p] = rest[r++];
     };
+    std::vector<int32_t> lf(static_cast<size_t>(n), -1);
+    fill(lf, 0, n, 1);
+    int rc = upload_order(ctx, lf, &scs->order_lf);
+    if (rc || in.C != kWave) return rc;
+    const int64_t per_block = kScsThreads / kWave;
+    const int64_t nb = ceil_div(n, per_block);
+    std::vector<int32_t> ord(static_cast<size_t>(nb * per_block), -1);
     if (scs->order_remap) {
         constexpr int64_t kXcd = 8;
         const int64_t q = nb / kXcd, rem = nb % kXcd;
         for (int64_t k = 0; k < kXcd; ++k) {
             const int64_t s = k < rem ? k * (q + 1) : rem * (q + 1) + (k - rem) * q;
             const int64_t len = k < rem ? q + 1 : q;
-            fill(s * per_block, (s + len) * per_block);
+            fill(ord, s * per_block, (s + len) * per_block, kLongSpread);
         }
     } else {
-        fill(0, nb * per_block);
+        fill(ord, 0, nb * per_block, kLongSpread);
     }
-    int rc = dev_alloc(ctx, reinterpret_cast<void **>(&scs->order),
-                       sizeof(int32_t) * ord.size(), "scs execution order");
-    if (rc) return rc;
-    SMAX_HIP_CALL(ctx, hipMemcpyAsync(scs->order, ord.data(), sizeof(int32_t) * ord.size(),
-                                      hipMemcpyHostToDevice, ctx->stream));
-    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
-    return SMAX_HIP_SUCCESS;
+    return upload_order(ctx, ord, &scs->order);
 }
 
 static int check_scs_args(smax_hip_ctx ctx, const char *fn, int value_type,
@@ -444,13 +540,14 @@ int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
         return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
                     "smax_hip_scs_apply: value type %d does not match %lld-byte values",
                     value_type, (long long)in.val_bytes);
-    const bool remap = in.C == kWave ? A->order_remap : true;
+    const bool w64 = in.C == kWave;
     SMAX_HIP_DISPATCH_VT(value_type,
                          return launch_scs_spmv<VT>(ctx, in.C, in.n_chunks, in.n_rows,
                                                     in.chunk_ptr, in.chunk_lengths,
                                                     in.col, in.val, in.slot_row, x, y,
-                                                    unpermuted != 0, remap,
-                                                    in.slot_len, A->order));
+                                                    unpermuted != 0, A->order_remap,
+                                                    in.slot_len,
+                                                    w64 ? A->order : A->order_lf));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 
@@ -467,19 +564,32 @@ int smax_hip_scs_spmm(smax_hip_ctx ctx, int value_type, int64_t C,
     SMAX_HIP_REQUIRE(ctx, slot_row && k > 0 && n_rows > 0 &&
                               n_rows <= n_chunks * C,
                      "smax_hip_scs_spmm: bad slot_row / n_rows / k");
-    const int64_t threads = n_chunks * C * k;
-    SMAX_HIP_DISPATCH_VT(value_type, {
-        using T = vt<VT>;
-        hipLaunchKernelGGL((scs_spmm_kernel<VT>), grid_for(threads, kScsThreads),
-                           dim3(kScsThreads), 0, ctx->stream, C, n_chunks,
-                           n_rows, k, chunk_ptr, chunk_lengths, col,
-                           static_cast<const typename T::val_t *>(val),
-                           slot_row,
-                           static_cast<const typename T::vec_t *>(X),
-                           static_cast<typename T::vec_t *>(Y));
-        SMAX_HIP_LAUNCHED(ctx, "scs_spmm_kernel");
-        return SMAX_HIP_SUCCESS;
-    });
+    SMAX_HIP_DISPATCH_VT(value_type,
+                         return launch_scs_spmm<VT>(ctx, C, n_chunks, n_rows, k, chunk_ptr,
+                                                    chunk_lengths, nullptr, nullptr, col,
+                                                    val, slot_row, X, Y));
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+int smax_hip_scs_apply_spmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
+                            int64_t k, const void *X, void *Y) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, A, "smax_hip_scs_apply_spmm: null matrix");
+    const smax_hip_scs_info &in = A->info;
+    int rc = check_scs_args(ctx, "smax_hip_scs_apply_spmm", value_type, in.C,
+                            in.n_chunks, in.chunk_ptr, in.chunk_lengths, in.col, in.val,
+                            X, Y);
+    if (rc) return rc;
+    SMAX_HIP_REQUIRE(ctx, k > 0, "smax_hip_scs_apply_spmm: k must be positive");
+    if (vt_val_bytes(value_type) != in.val_bytes)
+        return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                    "smax_hip_scs_apply_spmm: value type %d does not match %lld-byte "
+                    "values", value_type, (long long)in.val_bytes);
+    SMAX_HIP_DISPATCH_VT(value_type,
+                         return launch_scs_spmm<VT>(ctx, in.C, in.n_chunks, in.n_rows, k,
+                                                    in.chunk_ptr, in.chunk_lengths,
+                                                    in.slot_len, A->order_lf, in.col,
+                                                    in.val, in.slot_row, X, Y));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 

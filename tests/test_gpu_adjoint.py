@@ -109,7 +109,7 @@ class SpMMFunction(torch.autograd.Function):
         k = X.shape[1]
         op.T.refresh_values(vals)
         dX = torch.empty_like(X)
-        op.T.spmm(op.vt, k, dY, dX)
+        op.T.apply_spmm(op.vt, k, dY, dX)
         dvals = torch.empty_like(vals)
         op.S.sddmm(op.vt, k, dY, X, dvals)
         return dvals, dX, None

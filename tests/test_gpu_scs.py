@@ -264,6 +264,29 @@ def test_scs_spmm_bit_exact(ctx, vt, k):
     s.destroy()
 
 
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 4096), (32, 1 << 20), (4, 16)])
+@pytest.mark.parametrize("k", [1, 3, 32, 70])
+def test_scs_apply_spmm_equals_crs_spmm(ctx, vt, C_, sigma, k):
+    """Handle-level SpMM runs each output over the row's stored entries only:
+    it equals the reference CRS SpMM bit for bit even with inf/NaN in X's
+    row 0, which the padding terms of the padded chain would spread."""
+    A = ragged(6007, 5)
+    val, _ = values_for(vt, A)
+    X = (matgen.uniform_f64(5, A.n_cols * k) if vt == capi.F64
+         else matgen.uniform_f32(5, A.n_cols * k))
+    X[:k] = np.inf
+    X[1] = np.nan
+    s = gpu_scs(ctx, A, C_, sigma, val)
+    dX, dY = ctx.upload(X), ctx.zeros(A.n_rows * k, X.dtype)
+    s.apply_spmm(vt, k, dX, dY)
+    want = oracle.crs_spmm(A.row_ptr, A.col, val, k, True, X)
+    assert_same_values(ctx.download(dY), want, "SELL apply_spmm vs CRS SpMM")
+    for d in (dX, dY):
+        d.free()
+    s.destroy()
+
+
 def test_refresh_values(ctx):
     A = matgen.lomax_band(10000, half=1250)
     s = gpu_scs(ctx, A, 64, 128)
