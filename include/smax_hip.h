@@ -79,6 +79,7 @@ typedef enum {
 typedef struct smax_hip_ctx_s *smax_hip_ctx;
 typedef struct smax_hip_crs_plan_s *smax_hip_crs_plan;
 typedef struct smax_hip_scs_s *smax_hip_scs;
+typedef struct smax_hip_scs_plan_s *smax_hip_scs_plan;
 typedef struct smax_hip_bcrs_s *smax_hip_bcrs;
 
 /* ---------------------------------------------------------------- context */
@@ -188,6 +189,26 @@ int smax_hip_scs_spmv_unpermuted(smax_hip_ctx ctx, int value_type, int64_t C,
                                  const int32_t *col, const void *val,
                                  const int32_t *slot_row, const void *x,
                                  void *y);
+
+/*
+ * Execution schedule for smax_hip_scs_spmv_plan on caller-owned SELL arrays
+ * (the arrays the reference registers, spmv.hpp:37-50), built once from the
+ * device chunk_lengths: long chunks start first, and for sigma*16 <=
+ * n_rows_padded each XCD takes a contiguous range of chunks.  The plan holds
+ * only the schedule, so it stays valid while the arrays keep their chunk
+ * structure (new values are fine).  No reference counterpart.
+ */
+int smax_hip_scs_plan_create(smax_hip_ctx ctx, int64_t C, int64_t sigma,
+                             int64_t n_rows_padded, int64_t n_chunks,
+                             int64_t n_elements, const int32_t *chunk_lengths,
+                             smax_hip_scs_plan *out);
+int smax_hip_scs_plan_destroy(smax_hip_ctx ctx, smax_hip_scs_plan plan);
+
+/* smax_hip_scs_spmv through a plan: same output, bit for bit. */
+int smax_hip_scs_spmv_plan(smax_hip_ctx ctx, int value_type, smax_hip_scs_plan plan,
+                           const int32_t *chunk_ptr, const int32_t *chunk_lengths,
+                           const int32_t *col, const void *val, const void *x,
+                           void *y);
 
 /*
  * Y = A X on a SELL matrix, X (n_cols x k) and Y (n_rows x k) row-major, Y

@@ -94,6 +94,9 @@ _SIGS = {
                           C.c_int),
     "smax_hip_scs_spmv_unpermuted": ([vp, C.c_int, i64, i64, i64, vp, vp, vp,
                                       vp, vp, vp, vp], C.c_int),
+    "smax_hip_scs_plan_create": ([vp, i64, i64, i64, i64, i64, vp, vp], C.c_int),
+    "smax_hip_scs_plan_destroy": ([vp, vp], C.c_int),
+    "smax_hip_scs_spmv_plan": ([vp, C.c_int, vp, vp, vp, vp, vp, vp, vp], C.c_int),
     "smax_hip_scs_apply_spmm": ([vp, C.c_int, vp, i64, vp, vp], C.c_int),
     "smax_hip_scs_spmm": ([vp, C.c_int, i64, i64, i64, vp, vp, vp, vp, vp, i64,
                            vp, vp], C.c_int),
@@ -302,6 +305,22 @@ class Context:
         self._check(self.lib.smax_hip_scs_spmv(
             self.handle, vtype, C_, n_chunks, _ptr(chunk_ptr), _ptr(chunk_len),
             _ptr(col), _ptr(val), _ptr(x), _ptr(y)))
+
+    def scs_plan(self, C_, sigma, n_rows_padded, n_chunks, n_elements,
+                 chunk_lengths) -> int:
+        p = vp()
+        self._check(self.lib.smax_hip_scs_plan_create(
+            self.handle, C_, sigma, n_rows_padded, n_chunks, n_elements,
+            _ptr(chunk_lengths), C.byref(p)))
+        return p.value
+
+    def scs_plan_destroy(self, plan) -> None:
+        self._check(self.lib.smax_hip_scs_plan_destroy(self.handle, plan))
+
+    def scs_spmv_plan(self, vtype, plan, chunk_ptr, chunk_len, col, val, x, y) -> None:
+        self._check(self.lib.smax_hip_scs_spmv_plan(
+            self.handle, vtype, plan, _ptr(chunk_ptr), _ptr(chunk_len), _ptr(col),
+            _ptr(val), _ptr(x), _ptr(y)))
 
     def scs_from_crs(self, n_rows, n_cols, nnz, row_ptr, col, val, val_bytes,
                      C_, sigma) -> "Scs":

@@ -3,6 +3,17 @@
 
 #include "common.hpp"
 
+// Execution schedule of the SELL kernels, built once from the chunk lengths.
+struct smax_hip_scs_sched {
+    // C = 64 SpMV: wave position -> chunk (-1 = none), nullptr = chunk order;
+    // `remap` says whether it assumes the XCD-contiguous block remap.
+    int32_t *order = nullptr;
+    bool remap = false;
+    // SpMM and C != 64 SpMV: long chunks first, longest first, then the rest
+    // in chunk order; nullptr = chunk order.
+    int32_t *order_lf = nullptr;
+};
+
 struct smax_hip_scs_s {
     smax_hip_scs_info info;
     // Built on first use by the CG solvers, freed with the handle:
@@ -11,14 +22,12 @@ struct smax_hip_scs_s {
     uint16_t *val_bf16 = nullptr; // values rounded to bfloat16
     float *diag_f32 = nullptr;    // diagonal per slot, binary32
     int bf16_split = -1;          // 1: val_bf16 holds 0 on the diagonal
-    // Execution order of the C = 64 SpMV (block slot -> chunk, -1 = none;
-    // nullptr = chunk order) and whether it assumes the XCD-contiguous block
-    // remap.  Built with the conversion.
-    int32_t *order = nullptr;
-    bool order_remap = false;
-    // Chunk order of the SpMM kernels (any C): long chunks first, longest
-    // first, then the rest in chunk order; nullptr = chunk order.
-    int32_t *order_lf = nullptr;
+    smax_hip_scs_sched sched;     // built with the conversion
+};
+
+struct smax_hip_scs_plan_s {
+    smax_hip_scs_sched sched;
+    int64_t C = 0, n_chunks = 0;
 };
 
 namespace smaxhip {
@@ -33,7 +42,11 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
 
 void scs_free_arrays(smax_hip_scs_info &info);
 
-// Builds scs->order / scs->order_remap from the chunk lengths (synchronous).
-int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs);
+// Builds the execution schedule of a SELL layout from its device chunk
+// lengths (synchronous); free_schedule releases it.
+int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_padded,
+                   int64_t n_chunks, int64_t n_elements, const int32_t *chunk_lengths,
+                   smax_hip_scs_sched *out);
+void free_schedule(smax_hip_scs_sched &s);
 
 }  // namespace smaxhip

@@ -666,7 +666,8 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
     smax_hip_scs scs = new smax_hip_scs_s();
     scs->info = info;
     guard.keep = true;
-    rc = build_chunk_order(ctx, scs);
+    rc = build_schedule(ctx, C, sigma, n, n_chunks, n_elements, info.chunk_lengths,
+                        &scs->sched);
     if (rc) {
         smax_hip_scs_destroy(ctx, scs);
         return rc;
@@ -710,8 +711,7 @@ int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs) {
     dev_free(scs->val_f32);
     dev_free(scs->val_bf16);
     dev_free(scs->diag_f32);
-    dev_free(scs->order);
-    dev_free(scs->order_lf);
+    free_schedule(scs->sched);
     delete scs;
     return SMAX_HIP_SUCCESS;
 }

@@ -413,16 +413,25 @@ static int upload_order(smax_hip_ctx ctx, const std::vector<int32_t> &ord, int32
     return SMAX_HIP_SUCCESS;
 }
 
-int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs) {
-    const smax_hip_scs_info &in = scs->info;
-    scs->order_remap = in.sigma * 16 <= in.n_rows_padded;
-    const int64_t n = in.n_chunks;
+void free_schedule(smax_hip_scs_sched &s) {
+    dev_free(s.order);
+    dev_free(s.order_lf);
+    s.order = s.order_lf = nullptr;
+}
+
+int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_padded,
+                   int64_t n_chunks, int64_t n_elements, const int32_t *chunk_lengths,
+                   smax_hip_scs_sched *out) {
+    *out = smax_hip_scs_sched{};
+    out->remap = sigma * 16 <= n_rows_padded;
+    const int64_t n = n_chunks;
+    if (n <= 0) return SMAX_HIP_SUCCESS;
     std::vector<int32_t> cl(static_cast<size_t>(n));
-    SMAX_HIP_CALL(ctx, hipMemcpyAsync(cl.data(), in.chunk_lengths, sizeof(int32_t) * n,
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(cl.data(), chunk_lengths, sizeof(int32_t) * n,
                                       hipMemcpyDeviceToHost, ctx->stream));
     SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
     const double mean =
-        static_cast<double>(in.n_elements) / static_cast<double>(n * in.C);
+        static_cast<double>(n_elements) / static_cast<double>(n * C);
     const int64_t thr =
         std::max<int64_t>(kLongChunk, static_cast<int64_t>(std::ceil(8 * mean)));
     const int64_t n_long =
@@ -452,12 +461,12 @@ int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs) {
     };
     std::vector<int32_t> lf(static_cast<size_t>(n), -1);
     fill(lf, 0, n, 1);
-    int rc = upload_order(ctx, lf, &scs->order_lf);
-    if (rc || in.C != kWave) return rc;
+    int rc = upload_order(ctx, lf, &out->order_lf);
+    if (rc || C != kWave) return rc;
     const int64_t per_block = kScsThreads / kWave;
     const int64_t nb = ceil_div(n, per_block);
     std::vector<int32_t> ord(static_cast<size_t>(nb * per_block), -1);
-    if (scs->order_remap) {
+    if (out->remap) {
         constexpr int64_t kXcd = 8;
         const int64_t q = nb / kXcd, rem = nb % kXcd;
         for (int64_t k = 0; k < kXcd; ++k) {
@@ -468,7 +477,7 @@ int build_chunk_order(smax_hip_ctx ctx, smax_hip_scs scs) {
     } else {
         fill(ord, 0, nb * per_block, kLongSpread);
     }
-    return upload_order(ctx, ord, &scs->order);
+    return upload_order(ctx, ord, &out->order);
 }
 
 static int check_scs_args(smax_hip_ctx ctx, const char *fn, int value_type,
@@ -528,6 +537,62 @@ int smax_hip_scs_spmv_unpermuted(smax_hip_ctx ctx, int value_type, int64_t C,
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 
+int smax_hip_scs_plan_create(smax_hip_ctx ctx, int64_t C, int64_t sigma,
+                             int64_t n_rows_padded, int64_t n_chunks,
+                             int64_t n_elements, const int32_t *chunk_lengths,
+                             smax_hip_scs_plan *out) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, out, "smax_hip_scs_plan_create: null output");
+    *out = nullptr;
+    SMAX_HIP_REQUIRE(ctx, C > 0 && sigma > 0 && n_chunks > 0 && n_elements >= 0 &&
+                              n_rows_padded == n_chunks * C && chunk_lengths,
+                     "smax_hip_scs_plan_create: bad arguments");
+    auto *plan = new smax_hip_scs_plan_s();
+    plan->C = C;
+    plan->n_chunks = n_chunks;
+    int rc = build_schedule(ctx, C, sigma, n_rows_padded, n_chunks, n_elements,
+                            chunk_lengths, &plan->sched);
+    if (rc) {
+        free_schedule(plan->sched);
+        delete plan;
+        return rc;
+    }
+    *out = plan;
+    return SMAX_HIP_SUCCESS;
+}
+
+int smax_hip_scs_plan_destroy(smax_hip_ctx ctx, smax_hip_scs_plan plan) {
+    if (!plan) return SMAX_HIP_SUCCESS;
+    if (ctx) {
+        (void)bind(ctx);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    free_schedule(plan->sched);
+    delete plan;
+    return SMAX_HIP_SUCCESS;
+}
+
+int smax_hip_scs_spmv_plan(smax_hip_ctx ctx, int value_type, smax_hip_scs_plan plan,
+                           const int32_t *chunk_ptr, const int32_t *chunk_lengths,
+                           const int32_t *col, const void *val, const void *x,
+                           void *y) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, plan, "smax_hip_scs_spmv_plan: null plan");
+    int rc = check_scs_args(ctx, "smax_hip_scs_spmv_plan", value_type, plan->C,
+                            plan->n_chunks, chunk_ptr, chunk_lengths, col, val, x, y);
+    if (rc) return rc;
+    const bool w64 = plan->C == kWave;
+    const int64_t npad = plan->n_chunks * plan->C;
+    SMAX_HIP_DISPATCH_VT(value_type,
+                         return launch_scs_spmv<VT>(ctx, plan->C, plan->n_chunks, npad,
+                                                    chunk_ptr, chunk_lengths, col, val,
+                                                    nullptr, x, y, false,
+                                                    plan->sched.remap, nullptr,
+                                                    w64 ? plan->sched.order
+                                                        : plan->sched.order_lf));
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
 int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                        const void *x, void *y, int unpermuted) {
     SMAX_HIP_CHECK_CTX(ctx);
@@ -545,9 +610,10 @@ int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                          return launch_scs_spmv<VT>(ctx, in.C, in.n_chunks, in.n_rows,
                                                     in.chunk_ptr, in.chunk_lengths,
                                                     in.col, in.val, in.slot_row, x, y,
-                                                    unpermuted != 0, A->order_remap,
+                                                    unpermuted != 0, A->sched.remap,
                                                     in.slot_len,
-                                                    w64 ? A->order : A->order_lf));
+                                                    w64 ? A->sched.order
+                                                        : A->sched.order_lf));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 
@@ -588,7 +654,7 @@ int smax_hip_scs_apply_spmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
     SMAX_HIP_DISPATCH_VT(value_type,
                          return launch_scs_spmm<VT>(ctx, in.C, in.n_chunks, in.n_rows, k,
                                                     in.chunk_ptr, in.chunk_lengths,
-                                                    in.slot_len, A->order_lf, in.col,
+                                                    in.slot_len, A->sched.order_lf, in.col,
                                                     in.val, in.slot_row, X, Y));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }

@@ -44,8 +44,13 @@ struct DeviceState {
     smax_hip_crs_plan plan = nullptr;
     const void *plan_row_ptr = nullptr;  // host row_ptr the plan was built for
     ULL plan_rows = 0, plan_nnz = 0;
+    // SELL execution schedule, keyed like the CRS plan by the host array
+    smax_hip_scs_plan scs_plan = nullptr;
+    const void *scs_plan_lengths = nullptr;
+    ULL scs_plan_C = 0, scs_plan_sigma = 0, scs_plan_chunks = 0;
     ~DeviceState() {
         if (plan) smax_hip_crs_plan_destroy(stream.get(), plan);
+        if (scs_plan) smax_hip_scs_plan_destroy(stream.get(), scs_plan);
     }
 };
 
@@ -71,6 +76,21 @@ inline int initialize(Timers *timers, KernelContext *k, Args *args, Flags *flags
         rc |= d.a3.upload(ctx, A.val, vb * A.n_elements);
         x_size = A.n_cols;
         y_size = A.n_rows_padded;
+        if (!rc && (!d.scs_plan || d.scs_plan_lengths != A.chunk_lengths ||
+                    d.scs_plan_C != A.C || d.scs_plan_sigma != A.sigma ||
+                    d.scs_plan_chunks != A.n_chunks)) {
+            if (d.scs_plan) smax_hip_scs_plan_destroy(ctx, d.scs_plan);
+            d.scs_plan = nullptr;
+            rc |= report(ctx,
+                         smax_hip_scs_plan_create(ctx, A.C, A.sigma, A.n_rows_padded,
+                                                  A.n_chunks, A.n_elements,
+                                                  d.a1.as<int32_t>(), &d.scs_plan),
+                         "SELL execution plan");
+            d.scs_plan_lengths = A.chunk_lengths;
+            d.scs_plan_C = A.C;
+            d.scs_plan_sigma = A.sigma;
+            d.scs_plan_chunks = A.n_chunks;
+        }
     } else if (flags->is_mat_bcrs) {
         const BCRSMatrix &A = *args->A.bcrs;
         rc |= d.a0.upload(ctx, A.row_ptr, 4 * (A.n_rows + 1));
@@ -122,9 +142,13 @@ inline int apply(Timers *timers, KernelContext *k, Args *args, Flags *flags,
     int st;
     if (flags->is_mat_scs) {
         const SCSMatrix &A = *args->A.scs;
-        st = smax_hip_scs_spmv(ctx, vt, A.C, A.n_chunks, d.a0.as<int32_t>(),
-                               d.a1.as<int32_t>(), d.a2.as<int32_t>(), d.a3.get(),
-                               x, y);
+        st = d.scs_plan
+                 ? smax_hip_scs_spmv_plan(ctx, vt, d.scs_plan, d.a0.as<int32_t>(),
+                                          d.a1.as<int32_t>(), d.a2.as<int32_t>(),
+                                          d.a3.get(), x, y)
+                 : smax_hip_scs_spmv(ctx, vt, A.C, A.n_chunks, d.a0.as<int32_t>(),
+                                     d.a1.as<int32_t>(), d.a2.as<int32_t>(),
+                                     d.a3.get(), x, y);
     } else if (flags->is_mat_bcrs) {
         const BCRSMatrix &A = *args->A.bcrs;
         st = smax_hip_bcrs_spmv(ctx, vt, A.n_rows, A.b_height, A.b_width,

@@ -226,6 +226,27 @@ def test_scs_apply_matches_full_chunk_chains(ctx, vt, C_, sigma, x0):
     s.destroy()
 
 
+@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 256), (64, 1 << 20), (32, 512), (8, 1)])
+def test_scs_spmv_plan_on_caller_arrays(ctx, C_, sigma):
+    """The plan path (schedule only, caller-owned arrays) equals the raw SELL
+    SpMV bit for bit, with long chunks present (rows of 300)."""
+    A = ragged(9001, 17)
+    val, x = values_for(capi.F64, A)
+    s = gpu_scs(ctx, A, C_, sigma, val)
+    i = s.info
+    plan = ctx.scs_plan(C_, sigma, i.n_rows_padded, i.n_chunks, i.n_elements,
+                        i.chunk_lengths)
+    dx = ctx.upload(x)
+    y1, y2 = (ctx.zeros(i.n_rows_padded, np.float64) for _ in range(2))
+    s.spmv(capi.F64, dx, y1)
+    ctx.scs_spmv_plan(capi.F64, plan, i.chunk_ptr, i.chunk_lengths, i.col, i.val, dx, y2)
+    assert_same_values(ctx.download(y2), ctx.download(y1), "plan vs raw")
+    ctx.scs_plan_destroy(plan)
+    for d in (dx, y1, y2):
+        d.free()
+    s.destroy()
+
+
 def test_scs_apply_rejects_mismatched_value_type(ctx):
     A = matgen.poisson2d(16)
     s = gpu_scs(ctx, A, 64, 1)
