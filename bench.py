@@ -270,6 +270,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--full-sweep", action="store_true",
                     help="also time sigma in {1, 64} (17x padded, 13 GB each)")
+    ap.add_argument("--sigmas", default="4096,65536,N",
+                    help="comma list of SELL sigma values (N = n_rows)")
+    ap.add_argument("--cs", default="64,32", help="comma list of SELL chunk sizes")
     ap.add_argument("--only", default="",
                     help="comma list of sections to run: "
                          "stream,sell,crs,c1,cg,c4,c5,cpu")
@@ -301,8 +304,9 @@ def main() -> int:
     best = None
     sell = {}
     if want("sell"):
-        sigmas = [4096, 65536, A.n_rows] + ([1, 64] if args.full_sweep else [])
-        for C_ in (64, 32):
+        sigmas = [A.n_rows if t == "N" else int(t) for t in args.sigmas.split(",")]
+        sigmas += [1, 64] if args.full_sweep else []
+        for C_ in (int(t) for t in args.cs.split(",")):
             for sigma in sigmas:
                 t0 = time.perf_counter()
                 s = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 8,

@@ -95,6 +95,16 @@ inline bool valid_value_type(int v) {
     return v == SMAX_HIP_F64 || v == SMAX_HIP_F32 || v == SMAX_HIP_BF16;
 }
 
+// Workgroup order remapped so each XCD (workgroups are dealt round-robin to
+// the 8 XCDs) walks one contiguous range of work items: neighbouring items,
+// which gather overlapping parts of x, then share that XCD's L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks) {
+    constexpr int64_t kXcd = 8;
+    const int64_t xcd = bid % kXcd, local = bid / kXcd;
+    const int64_t q = nblocks / kXcd, r = nblocks % kXcd;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
 // Streamed-once loads bypass cache allocation.
 template <typename T> __device__ __forceinline__ T ld_stream(const T *p) {
     return __builtin_nontemporal_load(p);

@@ -13,7 +13,9 @@
 //    values and column indices through LDS in coalesced 1024-entry tiles
 //    (gathering x on the way), and each row's lane then runs its fma chain out
 //    of LDS.  Short rows are thereby reduced lane-per-row while the loads stay
-//    fully coalesced; a long row is streamed by its whole workgroup.
+//    fully coalesced; a long row is streamed by its whole workgroup.  Row
+//    blocks are dealt to the XCDs in contiguous ranges (xcd_block), so the x
+//    band of neighbouring blocks is shared in one L2.
 #include "common.hpp"
 #include "scan.hpp"
 
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmv_blocked(
     __shared__ A sv[kCrsTile];
     __shared__ A sx[kCrsTile];
     const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);
     const int64_t r0 = block_start[b], r1 = block_start[b + 1];
     const int64_t s = row_ptr[r0], e = row_ptr[r1];
     const int64_t row = r0 + tid;

@@ -23,16 +23,6 @@ constexpr int kScsThreads = 256;
 constexpr int kScsUnroll = 8;
 constexpr int kScsPipeUnroll = 16;
 
-// Workgroup order remapped so each XCD (workgroups are dealt round-robin to
-// the 8 XCDs) walks one contiguous range of chunks: neighbouring chunks, which
-// gather overlapping parts of x, then share that XCD's L2.
-__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks) {
-    constexpr int64_t kXcd = 8;
-    const int64_t xcd = bid % kXcd, local = bid / kXcd;
-    const int64_t q = nblocks / kXcd, r = nblocks % kXcd;
-    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
-}
-
 template <int VT, int U>
 __device__ __forceinline__ typename vt<VT>::vec_t
 slot_chain(const int32_t *__restrict__ col,
