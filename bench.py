@@ -397,10 +397,12 @@ def main() -> int:
         log(f"CPU baseline ({args.cpu_seconds} s sample)")
         base = cpu_baseline(A, x, args.cpu_seconds)
 
-    if best is None:
-        print("bench.py: no SpMV section ran (--only needs sell or crs)",
-              file=sys.stderr)
-        return 2
+    if best is None:  # a partial run (--only without sell/crs): extras only
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "GB/s",
+                          "n_gpus": 1, "extra": extra, "cpu_baseline": base}),
+              flush=True)
+        ctx.close()
+        return 0
     fmt, ms, kernel = best
     value = balg / ms / 1e6
     roof = {"bound": "hbm", "achieved": round(value, 1), "peak": HBM_PEAK_GBS,

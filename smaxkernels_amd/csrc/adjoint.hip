@@ -12,9 +12,8 @@
 // SDDMM: dvals[src_index[e]] = sum_v G[r*k+v] * X[col[e]*k+v] for every stored
 // element e of SELL row r; a group of up to 64 lanes takes one element, each
 // lane a strided part of v (fma chain), then a fixed xor-butterfly sums the
-// group: the reduction tree depends only on k.  Workgroups take contiguous
-// element ranges, so the chunk of an element is found by stepping, not by a
-// search per element.
+// group: the reduction tree depends only on k.  The row of each stored
+// element comes from a per-element row array built on first use.
 #include <utility>
 
 #include "common.hpp"
@@ -25,73 +24,73 @@
 namespace smaxhip {
 
 // ------------------------------------------------------------------ SDDMM
-// Workgroup b takes elements [b*seg, (b+1)*seg) in storage order; its groups
-// step through them, tracking the chunk of the current element (one binary
-// search over chunk_ptr per thread at the start, then forward steps).
+// elem_row[e] = original row of stored element e, -1 for padding; one lane
+// per slot walks its chunk column (stores coalesced across the chunk).
+__global__ __launch_bounds__(256) void k_elem_row(
+    int64_t C, int64_t n_slots, int64_t n_rows, const int32_t *__restrict__ chunk_ptr,
+    const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ slot_len,
+    const int32_t *__restrict__ slot_row, int32_t *__restrict__ elem_row) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (s >= n_slots) return;
+    const int64_t c = s / C, i = s - c * C;
+    const int64_t base = static_cast<int64_t>(chunk_ptr[c]) + i;
+    const int32_t r = slot_row[s];
+    const int len = r < n_rows ? slot_len[s] : 0;
+    const int clen = chunk_len[c];
+    for (int j = 0; j < clen; ++j) elem_row[base + static_cast<int64_t>(j) * C] = j < len ? r : -1;
+}
+
+// One group of G lanes per stored element.
 template <int VT, int G>
 __global__ __launch_bounds__(256) void scs_sddmm_kernel(
-    int64_t n_elements, int64_t C, int64_t n_chunks, int64_t k, int64_t seg,
-    const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ col,
-    const int32_t *__restrict__ slot_row, const int32_t *__restrict__ src,
+    int64_t n_elements, int64_t k, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ elem_row, const int32_t *__restrict__ src,
     const typename vt<VT>::vec_t *__restrict__ Gm,
     const typename vt<VT>::vec_t *__restrict__ X,
     typename vt<VT>::vec_t *__restrict__ dvals) {
     using T = vt<VT>;
     using A = typename T::vec_t;
-    constexpr int kGroups = 256 / G;
-    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * seg;
-    const int64_t e1 = e0 + seg < n_elements ? e0 + seg : n_elements;
-    const int group = threadIdx.x / G;
-    const int lane = threadIdx.x % G;
-    // chunk a with chunk_ptr[a] <= e0 < chunk_ptr[a+1] (empty chunks skipped)
-    int64_t a = 0, b = n_chunks;
-    while (b - a > 1) {
-        const int64_t m = (a + b) >> 1;
-        if (chunk_ptr[m] <= e0) a = m; else b = m;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t e = t / G;
+    const int lane = static_cast<int>(t % G);
+    const bool valid = e < n_elements;
+    const int32_t s = valid ? src[e] : -1;
+    A acc = 0;
+    if (s >= 0) {
+        const int64_t r = elem_row[e];
+        const int64_t c = col[e];
+        for (int64_t v = lane; v < k; v += G)
+            acc = T::fma_(Gm[r * k + v], X[c * k + v], acc);
     }
-    int64_t next = chunk_ptr[a + 1];
-    for (int64_t e = e0 + group; e < e1; e += kGroups) {
-        while (next <= e) next = chunk_ptr[++a + 1];
-        const int32_t s = src[e];
-        A acc = 0;
-        if (s >= 0) {
-            const int64_t off = e - chunk_ptr[a];
-            const int64_t slot = a * C + (C == kWave ? (off & (kWave - 1)) : off % C);
-            const int64_t r = slot_row[slot];
-            const int64_t c = col[e];
-            for (int64_t v = lane; v < k; v += G)
-                acc = T::fma_(Gm[r * k + v], X[c * k + v], acc);
-        }
 #pragma unroll
-        for (int off = G / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
-        if (s >= 0 && lane == 0) dvals[s] = acc;
-    }
+    for (int off = G / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
+    if (s >= 0 && lane == 0) dvals[s] = acc;
 }
 
 template <int VT, int G>
 static void launch_sddmm_g(smax_hip_ctx ctx, const smax_hip_scs_info &in,
-                           int64_t k, const void *Gm, const void *X, void *dv) {
+                           const int32_t *elem_row, int64_t k, const void *Gm,
+                           const void *X, void *dv) {
     using T = vt<VT>;
-    const int64_t seg = (256 / G) * 16;
-    hipLaunchKernelGGL((scs_sddmm_kernel<VT, G>), grid_for(in.n_elements, seg),
-                       dim3(256), 0, ctx->stream, in.n_elements, in.C,
-                       in.n_chunks, k, seg, in.chunk_ptr, in.col, in.slot_row,
+    hipLaunchKernelGGL((scs_sddmm_kernel<VT, G>), grid_for(in.n_elements * G, 256),
+                       dim3(256), 0, ctx->stream, in.n_elements, k, in.col, elem_row,
                        in.src_index, static_cast<const typename T::vec_t *>(Gm),
                        static_cast<const typename T::vec_t *>(X),
                        static_cast<typename T::vec_t *>(dv));
 }
 
 template <int VT>
-static int launch_sddmm(smax_hip_ctx ctx, const smax_hip_scs_info &in, int64_t k,
-                        int g, const void *Gm, const void *X, void *dv) {
+static int launch_sddmm(smax_hip_ctx ctx, const smax_hip_scs_info &in,
+                        const int32_t *elem_row, int64_t k, int g, const void *Gm,
+                        const void *X, void *dv) {
     switch (g) {
-    case 1: launch_sddmm_g<VT, 1>(ctx, in, k, Gm, X, dv); break;
-    case 2: launch_sddmm_g<VT, 2>(ctx, in, k, Gm, X, dv); break;
-    case 4: launch_sddmm_g<VT, 4>(ctx, in, k, Gm, X, dv); break;
-    case 8: launch_sddmm_g<VT, 8>(ctx, in, k, Gm, X, dv); break;
-    case 16: launch_sddmm_g<VT, 16>(ctx, in, k, Gm, X, dv); break;
-    case 32: launch_sddmm_g<VT, 32>(ctx, in, k, Gm, X, dv); break;
-    default: launch_sddmm_g<VT, 64>(ctx, in, k, Gm, X, dv); break;
+    case 1: launch_sddmm_g<VT, 1>(ctx, in, elem_row, k, Gm, X, dv); break;
+    case 2: launch_sddmm_g<VT, 2>(ctx, in, elem_row, k, Gm, X, dv); break;
+    case 4: launch_sddmm_g<VT, 4>(ctx, in, elem_row, k, Gm, X, dv); break;
+    case 8: launch_sddmm_g<VT, 8>(ctx, in, elem_row, k, Gm, X, dv); break;
+    case 16: launch_sddmm_g<VT, 16>(ctx, in, elem_row, k, Gm, X, dv); break;
+    case 32: launch_sddmm_g<VT, 32>(ctx, in, elem_row, k, Gm, X, dv); break;
+    default: launch_sddmm_g<VT, 64>(ctx, in, elem_row, k, Gm, X, dv); break;
     }
     SMAX_HIP_LAUNCHED(ctx, "scs_sddmm_kernel");
     return SMAX_HIP_SUCCESS;
@@ -162,10 +161,20 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                      "smax_hip_scs_sddmm: bad arguments");
     const smax_hip_scs_info &in = A->info;
     if (in.n_elements == 0) return SMAX_HIP_SUCCESS;
+    if (!A->elem_row) {
+        int rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->elem_row),
+                           sizeof(int32_t) * in.n_elements, "sddmm element rows");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_elem_row, grid_for(in.n_rows_padded, 256), dim3(256), 0,
+                           ctx->stream, in.C, in.n_rows_padded, in.n_rows, in.chunk_ptr,
+                           in.chunk_lengths, in.slot_len, in.slot_row, A->elem_row);
+        SMAX_HIP_LAUNCHED(ctx, "k_elem_row");
+    }
     int g = 1;
     while (g < k && g < 64) g <<= 1;
     SMAX_HIP_DISPATCH_VT(value_type,
-                         return launch_sddmm<VT>(ctx, in, k, g, G, X, dvals));
+                         return launch_sddmm<VT>(ctx, in, A->elem_row, k, g, G, X,
+                                                 dvals));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 

@@ -23,6 +23,9 @@ struct smax_hip_scs_s {
     float *diag_f32 = nullptr;    // diagonal per slot, binary32
     int bf16_split = -1;          // 1: val_bf16 holds 0 on the diagonal
     smax_hip_scs_sched sched;     // built with the conversion
+    // Built on first use by the SDDMM: row of every stored element (-1 for
+    // padding), so an element's row needs no search.
+    int32_t *elem_row = nullptr;
 };
 
 struct smax_hip_scs_plan_s {

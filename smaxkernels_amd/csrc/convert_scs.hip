@@ -712,6 +712,7 @@ int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs) {
     dev_free(scs->val_bf16);
     dev_free(scs->diag_f32);
     free_schedule(scs->sched);
+    dev_free(scs->elem_row);
     delete scs;
     return SMAX_HIP_SUCCESS;
 }
