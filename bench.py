@@ -112,14 +112,15 @@ def stream_peak(ctx, steps=10) -> dict:
             "array_GiB": 4}
 
 
-def hbm_traffic(kernel: str, n_rows: int):
-    """HBM bytes per launch of `kernel` on the N-row Lomax workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated against the
-    copy kernel), or None when that kernel and size were not profiled."""
+def hbm_traffic(kernel: str, n_rows: int, fmt: str):
+    """Bytes per launch of `kernel` between L2 and the fabric (HBM + MALL) on
+    the N-row Lomax workload in format `fmt`, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (calibrated against the copy kernel), or
+    None when that kernel, size and format were not profiled."""
     f = ROOT / "profiles" / "hbm_traffic.json"
     if not f.exists():
         return None
-    rec = json.loads(f.read_text()).get(f"{kernel}@lomax_band_N{n_rows}")
+    rec = json.loads(f.read_text()).get(f"{kernel}@lomax_band_N{n_rows}@{fmt}")
     return None if rec is None else rec["bytes_per_launch"]
 
 
@@ -407,7 +408,7 @@ def main() -> int:
     value = balg / ms / 1e6
     roof = {"bound": "hbm", "achieved": round(value, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(value / HBM_PEAK_GBS, 4),
-            "traffic": hbm_traffic(kernel, A.n_rows), "kernel": kernel}
+            "traffic": hbm_traffic(kernel, A.n_rows, fmt), "kernel": kernel}
     if peak_gbs:
         roof["frac_of_measured_stream"] = round(value / peak_gbs, 4)
         roof["measured_stream_GBs"] = peak_gbs
