@@ -193,15 +193,16 @@ int smax_hip_scs_spmv_unpermuted(smax_hip_ctx ctx, int value_type, int64_t C,
 /*
  * Execution schedule for smax_hip_scs_spmv_plan on caller-owned SELL arrays
  * (the arrays the reference registers, spmv.hpp:37-50), built once from the
- * device chunk_lengths: long chunks start first, and for sigma*16 <=
- * n_rows_padded each XCD takes a contiguous range of chunks.  The plan holds
+ * device chunk_lengths: long chunks start first, for sigma*16 <=
+ * n_rows_padded each XCD takes a contiguous range of chunks, and the fill
+ * n_elements / nnz decides the occupancy of the C = 64 kernel.  The plan holds
  * only the schedule, so it stays valid while the arrays keep their chunk
  * structure (new values are fine).  No reference counterpart.
  */
 int smax_hip_scs_plan_create(smax_hip_ctx ctx, int64_t C, int64_t sigma,
                              int64_t n_rows_padded, int64_t n_chunks,
-                             int64_t n_elements, const int32_t *chunk_lengths,
-                             smax_hip_scs_plan *out);
+                             int64_t n_elements, int64_t nnz,
+                             const int32_t *chunk_lengths, smax_hip_scs_plan *out);
 int smax_hip_scs_plan_destroy(smax_hip_ctx ctx, smax_hip_scs_plan plan);
 
 /* smax_hip_scs_spmv through a plan: same output, bit for bit. */

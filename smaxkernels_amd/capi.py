@@ -94,7 +94,7 @@ _SIGS = {
                           C.c_int),
     "smax_hip_scs_spmv_unpermuted": ([vp, C.c_int, i64, i64, i64, vp, vp, vp,
                                       vp, vp, vp, vp], C.c_int),
-    "smax_hip_scs_plan_create": ([vp, i64, i64, i64, i64, i64, vp, vp], C.c_int),
+    "smax_hip_scs_plan_create": ([vp, i64, i64, i64, i64, i64, i64, vp, vp], C.c_int),
     "smax_hip_scs_plan_destroy": ([vp, vp], C.c_int),
     "smax_hip_scs_spmv_plan": ([vp, C.c_int, vp, vp, vp, vp, vp, vp, vp], C.c_int),
     "smax_hip_scs_apply_spmm": ([vp, C.c_int, vp, i64, vp, vp], C.c_int),
@@ -306,11 +306,11 @@ class Context:
             self.handle, vtype, C_, n_chunks, _ptr(chunk_ptr), _ptr(chunk_len),
             _ptr(col), _ptr(val), _ptr(x), _ptr(y)))
 
-    def scs_plan(self, C_, sigma, n_rows_padded, n_chunks, n_elements,
+    def scs_plan(self, C_, sigma, n_rows_padded, n_chunks, n_elements, nnz,
                  chunk_lengths) -> int:
         p = vp()
         self._check(self.lib.smax_hip_scs_plan_create(
-            self.handle, C_, sigma, n_rows_padded, n_chunks, n_elements,
+            self.handle, C_, sigma, n_rows_padded, n_chunks, n_elements, nnz,
             _ptr(chunk_lengths), C.byref(p)))
         return p.value
 

@@ -145,6 +145,92 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_kernel(
     Y[(r * h_pad + ix) * k + v] = acc;
 }
 
+// Register-blocked SpMM for compile-time block shapes: thread (r, v) keeps
+// the BH outputs of block row r in column v, so each X row of a block is
+// loaded once for all BH rows (instead of once per row), and the BH chains
+// give independent fmas.  Per output the fma order is still the reference's
+// (block j, then jx); the next block's values and X entries are loaded while
+// this block's fmas run.
+template <int VT, int BH, int BW>
+__global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
+    int64_t n_block_rows, int h_pad, int w_pad, int col_major, int64_t k,
+    const int32_t *__restrict__ col, const int32_t *__restrict__ row_ptr,
+    const typename vt<VT>::val_t *__restrict__ val,
+    const typename vt<VT>::vec_t *__restrict__ X,
+    typename vt<VT>::vec_t *__restrict__ Y) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    using V = typename T::val_t;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
+    const int64_t r = t / k, v = t - r * k;
+    if (r >= n_block_rows) return;
+    const int64_t bs = static_cast<int64_t>(h_pad) * w_pad;
+    const int64_t vs_i = col_major ? 1 : w_pad, vs_j = col_major ? h_pad : 1;
+    A acc[BH];
+#pragma unroll
+    for (int ix = 0; ix < BH; ++ix) acc[ix] = 0;
+    int64_t j = row_ptr[r];
+    const int64_t je = row_ptr[r + 1];
+    if (j < je) {
+        V w0[BH][BW];
+        A x0[BW];
+        {
+            const V *blk = val + j * bs;
+            const A *xr = X + static_cast<int64_t>(col[j]) * w_pad * k + v;
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx) {
+                x0[jx] = xr[jx * k];
+#pragma unroll
+                for (int ix = 0; ix < BH; ++ix) w0[ix][jx] = blk[ix * vs_i + jx * vs_j];
+            }
+        }
+        for (; j + 1 < je; ++j) {
+            V w1[BH][BW];
+            A x1[BW];
+            const V *blk = val + (j + 1) * bs;
+            const A *xr = X + static_cast<int64_t>(col[j + 1]) * w_pad * k + v;
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx) {
+                x1[jx] = xr[jx * k];
+#pragma unroll
+                for (int ix = 0; ix < BH; ++ix) w1[ix][jx] = blk[ix * vs_i + jx * vs_j];
+            }
+#pragma unroll
+            for (int ix = 0; ix < BH; ++ix)
+#pragma unroll
+                for (int jx = 0; jx < BW; ++jx)
+                    acc[ix] = T::fma_(T::widen(w0[ix][jx]), x0[jx], acc[ix]);
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx) {
+                x0[jx] = x1[jx];
+#pragma unroll
+                for (int ix = 0; ix < BH; ++ix) w0[ix][jx] = w1[ix][jx];
+            }
+        }
+#pragma unroll
+        for (int ix = 0; ix < BH; ++ix)
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx)
+                acc[ix] = T::fma_(T::widen(w0[ix][jx]), x0[jx], acc[ix]);
+    }
+#pragma unroll
+    for (int ix = 0; ix < BH; ++ix) Y[(r * h_pad + ix) * k + v] = acc[ix];
+}
+
+template <int VT, int BH, int BW>
+static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t h_pad,
+                                 int64_t w_pad, bool cm, int64_t k, const int32_t *col,
+                                 const int32_t *row_ptr, const void *val, const void *X,
+                                 void *Y) {
+    using T = vt<VT>;
+    hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW>), grid_for(n_block_rows * k, kBThreads),
+                       dim3(kBThreads), 0, ctx->stream, n_block_rows,
+                       static_cast<int>(h_pad), static_cast<int>(w_pad), cm ? 1 : 0, k,
+                       col, row_ptr, static_cast<const typename T::val_t *>(val),
+                       static_cast<const typename T::vec_t *>(X),
+                       static_cast<typename T::vec_t *>(Y));
+}
+
 template <int VT, int BW>
 static void launch_bcrs_spmm(smax_hip_ctx ctx, int64_t threads, int64_t n_block_rows,
                              int64_t b_h, int64_t b_w, int64_t h_pad, int64_t w_pad,
@@ -315,6 +401,23 @@ int smax_hip_bcrs_spmm(smax_hip_ctx ctx, int value_type,
                      "smax_hip_bcrs_spmm: bad k or null pointer");
     const int64_t threads = n_block_rows * b_h * k;
     const bool cm = block_col_major != 0;
+    // register-blocked shapes: every (b_h, b_w) in {1, 2, 4} x {1, 2, 4}
+    const bool reg = (b_h == 1 || b_h == 2 || b_h == 4) && (b_w == 1 || b_w == 2 || b_w == 4);
+#define SMAX_BCRS_REG(H, W)                                                            \
+    if (b_h == H && b_w == W) {                                                        \
+        launch_bcrs_spmm_reg<VT, H, W>(ctx, n_block_rows, h_pad, w_pad, cm, k, col,    \
+                                       row_ptr, val, X, Y);                            \
+        SMAX_HIP_LAUNCHED(ctx, "bcrs_spmm_reg");                                       \
+        return SMAX_HIP_SUCCESS;                                                       \
+    }
+    if (reg) {
+        SMAX_HIP_DISPATCH_VT(value_type, {
+            SMAX_BCRS_REG(1, 1) SMAX_BCRS_REG(1, 2) SMAX_BCRS_REG(1, 4)
+            SMAX_BCRS_REG(2, 1) SMAX_BCRS_REG(2, 2) SMAX_BCRS_REG(2, 4)
+            SMAX_BCRS_REG(4, 1) SMAX_BCRS_REG(4, 2) SMAX_BCRS_REG(4, 4)
+        });
+    }
+#undef SMAX_BCRS_REG
     SMAX_HIP_DISPATCH_VT(value_type, {
         switch (b_w) {
         case 1:

@@ -12,6 +12,9 @@ struct smax_hip_scs_sched {
     // SpMM and C != 64 SpMV: long chunks first, longest first, then the rest
     // in chunk order; nullptr = chunk order.
     int32_t *order_lf = nullptr;
+    // Dynamic LDS reserved by the C = 64 SpMV launch to cap its occupancy
+    // (0 = no cap).
+    int64_t lds_cap_bytes = 0;
 };
 
 struct smax_hip_scs_s {
@@ -48,8 +51,8 @@ void scs_free_arrays(smax_hip_scs_info &info);
 // Builds the execution schedule of a SELL layout from its device chunk
 // lengths (synchronous); free_schedule releases it.
 int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_padded,
-                   int64_t n_chunks, int64_t n_elements, const int32_t *chunk_lengths,
-                   smax_hip_scs_sched *out);
+                   int64_t n_chunks, int64_t n_elements, int64_t nnz,
+                   const int32_t *chunk_lengths, smax_hip_scs_sched *out);
 void free_schedule(smax_hip_scs_sched &s);
 
 }  // namespace smaxhip

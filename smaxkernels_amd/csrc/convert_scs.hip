@@ -666,7 +666,7 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
     smax_hip_scs scs = new smax_hip_scs_s();
     scs->info = info;
     guard.keep = true;
-    rc = build_schedule(ctx, C, sigma, n, n_chunks, n_elements, info.chunk_lengths,
+    rc = build_schedule(ctx, C, sigma, n, n_chunks, n_elements, nnz, info.chunk_lengths,
                         &scs->sched);
     if (rc) {
         smax_hip_scs_destroy(ctx, scs);

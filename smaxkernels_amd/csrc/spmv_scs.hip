@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
         const int32_t r = slot_row[slot];
         if (r < n_rows) y[r] = acc;
     } else {
-        __builtin_nontemporal_store(acc, y + slot);
+        y[slot] = acc;
     }
 }
 
@@ -299,6 +299,7 @@ static int launch_scs_spmm(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_
 }
 
 struct W64Args {
+    int64_t lds_bytes;  // dynamic LDS reserved per workgroup (occupancy cap)
     int64_t n_chunks, n_rows;
     const int32_t *chunk_ptr, *chunk_len, *slot_len, *order, *col;
     const void *val;
@@ -311,8 +312,8 @@ template <int VT, bool UNPERM, bool REMAP, bool LEN>
 static void launch_w64(smax_hip_ctx ctx, const W64Args &a) {
     using T = vt<VT>;
     hipLaunchKernelGGL((scs_spmv_w64<VT, UNPERM, REMAP, LEN>),
-                       grid_for(a.n_chunks, kScsThreads / kWave), dim3(kScsThreads), 0,
-                       ctx->stream, a.n_chunks, a.n_rows, a.chunk_ptr, a.chunk_len,
+                       grid_for(a.n_chunks, kScsThreads / kWave), dim3(kScsThreads),
+                       static_cast<unsigned>(a.lds_bytes), ctx->stream, a.n_chunks, a.n_rows, a.chunk_ptr, a.chunk_len,
                        a.slot_len, a.order, a.col,
                        static_cast<const typename T::val_t *>(a.val), a.slot_row,
                        static_cast<const typename T::vec_t *>(a.x),
@@ -342,11 +343,11 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
                     const void *val, const int32_t *slot_row, const void *x,
                     void *y, bool unperm, bool remap = true,
                     const int32_t *slot_len = nullptr,
-                    const int32_t *order = nullptr) {
+                    const int32_t *order = nullptr, int64_t lds_bytes = 0) {
     using T = vt<VT>;
     if (C == kWave) {
-        const W64Args a{n_chunks, n_rows, chunk_ptr, chunk_len, slot_len, order,
-                        col, val, slot_row, x, y};
+        const W64Args a{lds_bytes, n_chunks, n_rows, chunk_ptr, chunk_len, slot_len,
+                        order, col, val, slot_row, x, y};
         if (unperm)
             launch_w64_sched<VT, true>(ctx, remap, a);
         else
@@ -391,6 +392,7 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
 // the range has room (each on its own CU, with its own miss queue); the rest
 // follow in chunk order.  Results do not depend on the order.
 constexpr int kLongChunk = 128;
+constexpr int64_t kLdsPerCu = 160 * 1024;
 constexpr int kLongSpread = kScsThreads / kWave;
 
 static int upload_order(smax_hip_ctx ctx, const std::vector<int32_t> &ord, int32_t **out) {
@@ -410,10 +412,17 @@ void free_schedule(smax_hip_scs_sched &s) {
 }
 
 int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_padded,
-                   int64_t n_chunks, int64_t n_elements, const int32_t *chunk_lengths,
-                   smax_hip_scs_sched *out) {
+                   int64_t n_chunks, int64_t n_elements, int64_t nnz,
+                   const int32_t *chunk_lengths, smax_hip_scs_sched *out) {
     *out = smax_hip_scs_sched{};
     out->remap = sigma * 16 <= n_rows_padded;
+    // Occupancy: with little padding every lane of a wave has gathers in
+    // flight, and five waves per SIMD congest the path from L2 to the MALL;
+    // reserving LDS for four workgroups per CU (four waves per SIMD) runs
+    // faster (0.70 -> 0.63 ms on the 4M-row Lomax matrix at sigma = 65536).
+    // Padded layouts (fill > 1.5) keep lanes idle and want every wave.
+    if (C == kWave && nnz > 0 && 2 * n_elements <= 3 * nnz)
+        out->lds_cap_bytes = kLdsPerCu / 4;
     const int64_t n = n_chunks;
     if (n <= 0) return SMAX_HIP_SUCCESS;
     std::vector<int32_t> cl(static_cast<size_t>(n));
@@ -529,18 +538,19 @@ int smax_hip_scs_spmv_unpermuted(smax_hip_ctx ctx, int value_type, int64_t C,
 
 int smax_hip_scs_plan_create(smax_hip_ctx ctx, int64_t C, int64_t sigma,
                              int64_t n_rows_padded, int64_t n_chunks,
-                             int64_t n_elements, const int32_t *chunk_lengths,
-                             smax_hip_scs_plan *out) {
+                             int64_t n_elements, int64_t nnz,
+                             const int32_t *chunk_lengths, smax_hip_scs_plan *out) {
     SMAX_HIP_CHECK_CTX(ctx);
     SMAX_HIP_REQUIRE(ctx, out, "smax_hip_scs_plan_create: null output");
     *out = nullptr;
     SMAX_HIP_REQUIRE(ctx, C > 0 && sigma > 0 && n_chunks > 0 && n_elements >= 0 &&
+                              nnz >= 0 && nnz <= n_elements &&
                               n_rows_padded == n_chunks * C && chunk_lengths,
                      "smax_hip_scs_plan_create: bad arguments");
     auto *plan = new smax_hip_scs_plan_s();
     plan->C = C;
     plan->n_chunks = n_chunks;
-    int rc = build_schedule(ctx, C, sigma, n_rows_padded, n_chunks, n_elements,
+    int rc = build_schedule(ctx, C, sigma, n_rows_padded, n_chunks, n_elements, nnz,
                             chunk_lengths, &plan->sched);
     if (rc) {
         free_schedule(plan->sched);
@@ -579,7 +589,8 @@ int smax_hip_scs_spmv_plan(smax_hip_ctx ctx, int value_type, smax_hip_scs_plan p
                                                     nullptr, x, y, false,
                                                     plan->sched.remap, nullptr,
                                                     w64 ? plan->sched.order
-                                                        : plan->sched.order_lf));
+                                                        : plan->sched.order_lf,
+                                                    plan->sched.lds_cap_bytes));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 
@@ -603,7 +614,8 @@ int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                                                     unpermuted != 0, A->sched.remap,
                                                     in.slot_len,
                                                     w64 ? A->sched.order
-                                                        : A->sched.order_lf));
+                                                        : A->sched.order_lf,
+                                                    A->sched.lds_cap_bytes));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 

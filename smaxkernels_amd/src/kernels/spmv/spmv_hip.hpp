@@ -83,7 +83,7 @@ inline int initialize(Timers *timers, KernelContext *k, Args *args, Flags *flags
             d.scs_plan = nullptr;
             rc |= report(ctx,
                          smax_hip_scs_plan_create(ctx, A.C, A.sigma, A.n_rows_padded,
-                                                  A.n_chunks, A.n_elements,
+                                                  A.n_chunks, A.n_elements, A.nnz,
                                                   d.a1.as<int32_t>(), &d.scs_plan),
                          "SELL execution plan");
             d.scs_plan_lengths = A.chunk_lengths;

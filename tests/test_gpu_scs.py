@@ -234,7 +234,7 @@ def test_scs_spmv_plan_on_caller_arrays(ctx, C_, sigma):
     val, x = values_for(capi.F64, A)
     s = gpu_scs(ctx, A, C_, sigma, val)
     i = s.info
-    plan = ctx.scs_plan(C_, sigma, i.n_rows_padded, i.n_chunks, i.n_elements,
+    plan = ctx.scs_plan(C_, sigma, i.n_rows_padded, i.n_chunks, i.n_elements, A.nnz,
                         i.chunk_lengths)
     dx = ctx.upload(x)
     y1, y2 = (ctx.zeros(i.n_rows_padded, np.float64) for _ in range(2))
