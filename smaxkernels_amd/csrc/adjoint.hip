@@ -12,8 +12,10 @@
 // SDDMM: dvals[src_index[e]] = sum_v G[r*k+v] * X[col[e]*k+v] for every stored
 // element e of SELL row r; a group of up to 64 lanes takes one element, each
 // lane a strided part of v (fma chain), then a fixed xor-butterfly sums the
-// group: the reduction tree depends only on k.  The row of each stored
-// element comes from a per-element row array built on first use.
+// group: the reduction tree depends only on k.  For k a power of two up to
+// 64 the lanes of a group share one row (SpMM order), so the G row is read
+// once per row; other k take one group per stored element, with the row of
+// each element from a per-element row array built on first use.
 #include <utility>
 
 #include "common.hpp"
@@ -65,6 +67,104 @@ __global__ __launch_bounds__(256) void scs_sddmm_kernel(
 #pragma unroll
     for (int off = G / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
     if (s >= 0 && lane == 0) dvals[s] = acc;
+}
+
+// k a power of two <= 64: k lanes per slot (row) in SpMM order, long
+// chunks first.  Lane v keeps G[r, v] in a register for the whole row and
+// each element costs one coalesced X row; per element the arithmetic is the
+// kernel above's (one fma per lane, then the xor-butterfly), so dvals are
+// bit-identical to it.
+template <int VT, int K>
+__global__ __launch_bounds__(256) void scs_sddmm_rows(
+    int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
+    const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ slot_len,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ slot_row, const int32_t *__restrict__ src,
+    const typename vt<VT>::vec_t *__restrict__ Gm,
+    const typename vt<VT>::vec_t *__restrict__ X,
+    typename vt<VT>::vec_t *__restrict__ dvals) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    const int64_t ck = C * K;
+    int64_t pos, li;
+    if (cpb > 1) {
+        if (threadIdx.x >= cpb * ck) return;
+        pos = static_cast<int64_t>(blockIdx.x) * cpb + threadIdx.x / ck;
+        li = threadIdx.x % ck;
+    } else {
+        pos = blockIdx.x / bpc;
+        li = (blockIdx.x % bpc) * 256 + threadIdx.x;
+    }
+    if (pos >= n_chunks || li >= ck) return;
+    const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
+    const int64_t i = li / K;
+    const int v = static_cast<int>(li % K);
+    const int64_t slot = chunk * C + i;
+    const int32_t r = slot_row[slot];
+    if (r >= n_rows) return;
+    const int len = slot_len[slot];
+    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
+    const A g = Gm[static_cast<int64_t>(r) * K + v];
+    constexpr int U = 4;
+    int j = 0;
+    for (; j + U <= len; j += U) {
+        int32_t c[U], sv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e = base + static_cast<int64_t>(j + u) * C;
+            c[u] = col[e];
+            sv[u] = src[e];
+        }
+        A xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = X[static_cast<int64_t>(c[u]) * K + v];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            A acc = T::fma_(g, xv[u], A(0));
+#pragma unroll
+            for (int off = K / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, K);
+            if (v == 0) dvals[sv[u]] = acc;
+        }
+    }
+    for (; j < len; ++j) {
+        const int64_t e = base + static_cast<int64_t>(j) * C;
+        A acc = T::fma_(g, X[static_cast<int64_t>(col[e]) * K + v], A(0));
+#pragma unroll
+        for (int off = K / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, K);
+        if (v == 0) dvals[src[e]] = acc;
+    }
+}
+
+template <int VT, int K>
+static void launch_sddmm_rows(smax_hip_ctx ctx, smax_hip_scs A, const void *Gm,
+                              const void *X, void *dv) {
+    using T = vt<VT>;
+    const smax_hip_scs_info &in = A->info;
+    const int64_t ck = in.C * K;
+    const int64_t cpb = ck <= 256 ? 256 / ck : 1;
+    const int64_t bpc = ck <= 256 ? 1 : ceil_div(ck, 256);
+    const int64_t blocks = cpb > 1 ? ceil_div(in.n_chunks, cpb) : in.n_chunks * bpc;
+    hipLaunchKernelGGL((scs_sddmm_rows<VT, K>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(256), 0, ctx->stream, in.C, in.n_chunks, in.n_rows, cpb, bpc,
+                       in.chunk_ptr, in.slot_len, A->sched.order_lf, in.col, in.slot_row,
+                       in.src_index, static_cast<const typename T::vec_t *>(Gm),
+                       static_cast<const typename T::vec_t *>(X),
+                       static_cast<typename T::vec_t *>(dv));
+}
+
+template <int VT>
+static bool sddmm_rows(smax_hip_ctx ctx, smax_hip_scs A, int64_t k, const void *Gm,
+                       const void *X, void *dv) {
+    switch (k) {
+    case 1: launch_sddmm_rows<VT, 1>(ctx, A, Gm, X, dv); return true;
+    case 2: launch_sddmm_rows<VT, 2>(ctx, A, Gm, X, dv); return true;
+    case 4: launch_sddmm_rows<VT, 4>(ctx, A, Gm, X, dv); return true;
+    case 8: launch_sddmm_rows<VT, 8>(ctx, A, Gm, X, dv); return true;
+    case 16: launch_sddmm_rows<VT, 16>(ctx, A, Gm, X, dv); return true;
+    case 32: launch_sddmm_rows<VT, 32>(ctx, A, Gm, X, dv); return true;
+    case 64: launch_sddmm_rows<VT, 64>(ctx, A, Gm, X, dv); return true;
+    default: return false;
+    }
 }
 
 template <int VT, int G>
@@ -161,6 +261,14 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                      "smax_hip_scs_sddmm: bad arguments");
     const smax_hip_scs_info &in = A->info;
     if (in.n_elements == 0) return SMAX_HIP_SUCCESS;
+    if (in.C * k <= INT32_MAX && (k & (k - 1)) == 0 && k <= 64) {
+        bool done = false;
+        SMAX_HIP_DISPATCH_VT(value_type, done = sddmm_rows<VT>(ctx, A, k, G, X, dvals));
+        if (done) {
+            SMAX_HIP_LAUNCHED(ctx, "scs_sddmm_rows");
+            return SMAX_HIP_SUCCESS;
+        }
+    }
     if (!A->elem_row) {
         int rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->elem_row),
                            sizeof(int32_t) * in.n_elements, "sddmm element rows");

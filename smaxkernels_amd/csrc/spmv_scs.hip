@@ -212,6 +212,7 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmm_kernel(
     const int64_t ck = C * k;
     int64_t pos, li;
     if (cpb > 1) {
+        if (threadIdx.x >= cpb * ck) return;
         pos = static_cast<int64_t>(blockIdx.x) * cpb + threadIdx.x / ck;
         li = threadIdx.x % ck;
     } else {

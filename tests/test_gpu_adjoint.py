@@ -51,15 +51,19 @@ def test_transpose_refresh_follows_new_values(ctx):
     T.destroy()
 
 
-@pytest.mark.parametrize("vt,k", [(capi.F64, 1), (capi.F64, 5), (capi.F64, 32),
-                                  (capi.F32, 1), (capi.F32, 32), (capi.BF16, 32)])
-def test_sddmm(ctx, vt, k):
+@pytest.mark.parametrize("vt,k,C_", [(capi.F64, 1, 64), (capi.F64, 5, 64), (capi.F64, 32, 64),
+                                     (capi.F32, 1, 64), (capi.F32, 32, 64), (capi.BF16, 32, 64),
+                                     (capi.F32, 8, 7), (capi.F32, 64, 64), (capi.F64, 3, 32),
+                                     (capi.F32, 16, 3)])
+def test_sddmm(ctx, vt, k, C_):
+    """k a power of two takes the row-ordered kernel, other k the per-element
+    one; C = 3 and 7 leave workgroups partly unused."""
     A = matgen.lomax_band(20000, half=2500)
     val = {capi.F64: A.val, capi.F32: A.val.astype(np.float32),
            capi.BF16: matgen.to_bf16(A.val)}[vt]
     rp, col, v = upload_crs(ctx, A, val)
     S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, val.dtype.itemsize,
-                         64, 512)
+                         C_, 512)
     vec = np.float64 if vt == capi.F64 else np.float32
     G = matgen.uniform_f64(6, A.n_rows * k).astype(vec)
     X = matgen.uniform_f64(7, A.n_cols * k).astype(vec)
