@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
 // Any C: one lane per slot, 256 / C chunks per workgroup (C <= 256) or one
 // chunk over several workgroups; chunks in the optional execution `order`;
 // LEN as in scs_spmv_w64.
-template <int VT, bool UNPERM, bool LEN>
+template <int VT, bool UNPERM, bool LEN, bool REMAP>
 __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
     int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
     const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ chunk_len,
@@ -165,7 +165,8 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
     using T = vt<VT>;
     int64_t pos, i;
     if (cpb > 0) {
-        pos = static_cast<int64_t>(blockIdx.x) * cpb + threadIdx.x / C;
+        const int64_t blk = REMAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+        pos = blk * cpb + threadIdx.x / C;
         i = threadIdx.x % C;
         if (threadIdx.x >= cpb * C) return;
     } else {
@@ -175,6 +176,7 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
     }
     if (pos >= n_chunks) return;
     const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
+    if (chunk < 0) return;
     const int64_t slot = chunk * C + i;
     const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
     const int clen = chunk_len[chunk];
@@ -364,14 +366,20 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
     const int64_t blocks = cpb > 0 ? ceil_div(n_chunks, cpb) : n_chunks * bpc;
     SMAX_HIP_REQUIRE(ctx, blocks <= INT32_MAX, "scs spmv: grid too large");
     const dim3 grid(static_cast<unsigned>(blocks));
-#define SMAX_SCS_GENERIC(U, L)                                                       \
-    hipLaunchKernelGGL((scs_spmv_generic<VT, U, L>), grid, dim3(kScsThreads), 0,    \
+    const bool rm = remap && cpb > 0;
+#define SMAX_SCS_GENERIC(U, L, R)                                                    \
+    hipLaunchKernelGGL((scs_spmv_generic<VT, U, L, R>), grid, dim3(kScsThreads), 0, \
                        ctx->stream, C, n_chunks, n_rows, cpb, bpc, chunk_ptr,       \
                        chunk_len, slot_len, order, col, v, slot_row, xv, yv)
-    if (unperm && slot_len) SMAX_SCS_GENERIC(true, true);
-    else if (unperm) SMAX_SCS_GENERIC(true, false);
-    else if (slot_len) SMAX_SCS_GENERIC(false, true);
-    else SMAX_SCS_GENERIC(false, false);
+    if (unperm && slot_len) {
+        if (rm) SMAX_SCS_GENERIC(true, true, true); else SMAX_SCS_GENERIC(true, true, false);
+    } else if (unperm) {
+        if (rm) SMAX_SCS_GENERIC(true, false, true); else SMAX_SCS_GENERIC(true, false, false);
+    } else if (slot_len) {
+        if (rm) SMAX_SCS_GENERIC(false, true, true); else SMAX_SCS_GENERIC(false, true, false);
+    } else {
+        if (rm) SMAX_SCS_GENERIC(false, false, true); else SMAX_SCS_GENERIC(false, false, false);
+    }
 #undef SMAX_SCS_GENERIC
     SMAX_HIP_LAUNCHED(ctx, "scs_spmv_generic");
     return SMAX_HIP_SUCCESS;
@@ -394,7 +402,6 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
 // follow in chunk order.  Results do not depend on the order.
 constexpr int kLongChunk = 128;
 constexpr int64_t kLdsPerCu = 160 * 1024;
-constexpr int kLongSpread = kScsThreads / kWave;
 
 static int upload_order(smax_hip_ctx ctx, const std::vector<int32_t> &ord, int32_t **out) {
     int rc = dev_alloc(ctx, reinterpret_cast<void **>(out), sizeof(int32_t) * ord.size(),
@@ -462,8 +469,8 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
     std::vector<int32_t> lf(static_cast<size_t>(n), -1);
     fill(lf, 0, n, 1);
     int rc = upload_order(ctx, lf, &out->order_lf);
-    if (rc || C != kWave) return rc;
-    const int64_t per_block = kScsThreads / kWave;
+    if (rc || C > kScsThreads) return rc;
+    const int64_t per_block = kScsThreads / C;  // chunks per workgroup
     const int64_t nb = ceil_div(n, per_block);
     std::vector<int32_t> ord(static_cast<size_t>(nb * per_block), -1);
     if (out->remap) {
@@ -472,10 +479,10 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
         for (int64_t k = 0; k < kXcd; ++k) {
             const int64_t s = k < rem ? k * (q + 1) : rem * (q + 1) + (k - rem) * q;
             const int64_t len = k < rem ? q + 1 : q;
-            fill(ord, s * per_block, (s + len) * per_block, kLongSpread);
+            fill(ord, s * per_block, (s + len) * per_block, per_block);
         }
     } else {
-        fill(ord, 0, nb * per_block, kLongSpread);
+        fill(ord, 0, nb * per_block, per_block);
     }
     return upload_order(ctx, ord, &out->order);
 }
@@ -582,14 +589,14 @@ int smax_hip_scs_spmv_plan(smax_hip_ctx ctx, int value_type, smax_hip_scs_plan p
     int rc = check_scs_args(ctx, "smax_hip_scs_spmv_plan", value_type, plan->C,
                             plan->n_chunks, chunk_ptr, chunk_lengths, col, val, x, y);
     if (rc) return rc;
-    const bool w64 = plan->C == kWave;
     const int64_t npad = plan->n_chunks * plan->C;
     SMAX_HIP_DISPATCH_VT(value_type,
                          return launch_scs_spmv<VT>(ctx, plan->C, plan->n_chunks, npad,
                                                     chunk_ptr, chunk_lengths, col, val,
                                                     nullptr, x, y, false,
                                                     plan->sched.remap, nullptr,
-                                                    w64 ? plan->sched.order
+                                                    plan->C <= kScsThreads
+                                                        ? plan->sched.order
                                                         : plan->sched.order_lf,
                                                     plan->sched.lds_cap_bytes));
     return SMAX_HIP_ERROR_UNSUPPORTED;
@@ -607,14 +614,14 @@ int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
         return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
                     "smax_hip_scs_apply: value type %d does not match %lld-byte values",
                     value_type, (long long)in.val_bytes);
-    const bool w64 = in.C == kWave;
     SMAX_HIP_DISPATCH_VT(value_type,
                          return launch_scs_spmv<VT>(ctx, in.C, in.n_chunks, in.n_rows,
                                                     in.chunk_ptr, in.chunk_lengths,
                                                     in.col, in.val, in.slot_row, x, y,
                                                     unpermuted != 0, A->sched.remap,
                                                     in.slot_len,
-                                                    w64 ? A->sched.order
+                                                    in.C <= kScsThreads
+                                                        ? A->sched.order
                                                         : A->sched.order_lf,
                                                     A->sched.lds_cap_bytes));
     return SMAX_HIP_ERROR_UNSUPPORTED;

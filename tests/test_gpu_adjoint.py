@@ -69,18 +69,26 @@ def test_sddmm(ctx, vt, k, C_):
     X = matgen.uniform_f64(7, A.n_cols * k).astype(vec)
     dv = ctx.zeros(A.nnz, vec)
     S.sddmm(vt, k, ctx.upload(G), ctx.upload(X), dv)
-    got = ctx.download(dv).astype(np.float64)
+    got = ctx.download(dv)
     want = oracle.sddmm_f64(A.row_ptr, A.col, k, G.astype(np.float64),
                             X.astype(np.float64))
     rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
-    scale = np.abs(G.astype(np.float64).reshape(-1, k)[rows] *
-                   X.astype(np.float64).reshape(-1, k)[A.col]).sum(axis=1)
+    # exact enough for an FP64 kernel too: x87 extended precision (64-bit
+    # significand) products and sums
+    Gl = G.astype(np.longdouble).reshape(-1, k)[rows]
+    Xl = X.astype(np.longdouble).reshape(-1, k)[A.col]
+    exact = (Gl * Xl).sum(axis=1)
+    scale = np.abs(Gl * Xl).sum(axis=1)
     if k == 1:
         # one product, rounded once
-        assert_bitwise(got, (G[rows] * X[A.col]).astype(np.float64), "k=1")
+        assert_bitwise(got, G[rows] * X[A.col], "k=1")
     u = np.finfo(vec).eps / 2
-    bound = (np.ceil(np.log2(k)) + 1) * u * scale + 1e-300
-    assert (np.abs(got - want) <= bound).all()
+    bound = (np.ceil(np.log2(k)) + 1) * u * scale * (1 + 4 * u) + 1e-300
+    assert (np.abs(got.astype(np.longdouble) - exact) <= bound).all()
+    # the oracle's FP64 sums agree within their own rounding and ours
+    assert (np.abs(got.astype(np.float64) - want) <=
+            (k + np.ceil(np.log2(k)) + 2) * np.finfo(np.float64).eps * scale.astype(np.float64)
+            + (u if vec == np.float32 else 0) * 8 * scale.astype(np.float64) + 1e-300).all()
     S.destroy()
 
 
