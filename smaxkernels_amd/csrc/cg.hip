@@ -52,6 +52,53 @@ __device__ __forceinline__ W block_sum(W v) {
     return t;
 }
 
+// Sum over a workgroup of NT threads (fixed tree: wave butterflies, then the
+// wave sums in wave order).
+template <typename W, int NT>
+__device__ __forceinline__ W block_sum_n(W v) {
+    __shared__ W ws[NT / kWave];
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    const int wave = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[wave] = v;
+    __syncthreads();
+    W t = ws[0];
+#pragma unroll
+    for (int w = 1; w < NT / kWave; ++w) t += ws[w];
+    __syncthreads();
+    return t;
+}
+
+// Reductions of per-block partials run in one workgroup of kReduceThreads;
+// each thread issues kReduceUnroll loads before adding them (independent
+// accumulators summed in a fixed order), so the pass is not a chain of
+// dependent memory round trips.
+constexpr int kReduceThreads = 1024;
+constexpr int kReduceUnroll = 8;
+
+template <typename W>
+__device__ __forceinline__ W reduce_partials(const W *__restrict__ partial, int64_t m) {
+    W a[kReduceUnroll];
+#pragma unroll
+    for (int u = 0; u < kReduceUnroll; ++u) a[u] = 0;
+    int64_t i = threadIdx.x;
+    constexpr int64_t kStep = static_cast<int64_t>(kReduceThreads) * kReduceUnroll;
+    for (; i + (kReduceUnroll - 1) * kReduceThreads < m; i += kStep) {
+        W v[kReduceUnroll];
+#pragma unroll
+        for (int u = 0; u < kReduceUnroll; ++u) v[u] = partial[i + u * kReduceThreads];
+#pragma unroll
+        for (int u = 0; u < kReduceUnroll; ++u) a[u] += v[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kReduceUnroll; ++u)
+        if (i + u * kReduceThreads < m) a[u] += partial[i + u * kReduceThreads];
+    W t = a[0];
+#pragma unroll
+    for (int u = 1; u < kReduceUnroll; ++u) t += a[u];
+    return block_sum_n<W, kReduceThreads>(t);
+}
+
 template <typename V> struct widen_t;
 template <> struct widen_t<double> {
     __device__ static double f(double v) { return v; }
@@ -152,14 +199,12 @@ __global__ __launch_bounds__(kCgThreads) void k_residual(
 // *out = sum of partial[0..m) in a fixed order.  With ctl, skipped once the
 // loop is done, and (stop2 >= 0) raises the done flag when *out <= stop2.
 template <typename W>
-__global__ __launch_bounds__(kCgThreads) void k_reduce(const W *__restrict__ partial,
-                                                       int64_t m, W *__restrict__ out,
-                                                       int64_t *__restrict__ ctl,
-                                                       W stop2) {
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(const W *__restrict__ partial,
+                                                           int64_t m, W *__restrict__ out,
+                                                           int64_t *__restrict__ ctl,
+                                                           W stop2) {
     if (ctl && ctl[CTL_DONE]) return;
-    W s = 0;
-    for (int64_t i = threadIdx.x; i < m; i += kCgThreads) s += partial[i];
-    const W t = block_sum(s);
+    const W t = reduce_partials(partial, m);
     if (threadIdx.x == 0) {
         *out = t;
         if (ctl && stop2 >= W(0) && !(t > stop2)) ctl[CTL_DONE] = 1;
@@ -170,13 +215,11 @@ __global__ __launch_bounds__(kCgThreads) void k_reduce(const W *__restrict__ par
 // is not positive (breakdown: the operator is not positive definite in the
 // working precision, or a NaN).
 template <typename W>
-__global__ __launch_bounds__(kCgThreads) void k_reduce_pq(const W *__restrict__ partial,
-                                                          int64_t m, W *__restrict__ out,
-                                                          int64_t *__restrict__ ctl) {
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_pq(const W *__restrict__ partial,
+                                                              int64_t m, W *__restrict__ out,
+                                                              int64_t *__restrict__ ctl) {
     if (ctl[CTL_DONE]) return;
-    W s = 0;
-    for (int64_t i = threadIdx.x; i < m; i += kCgThreads) s += partial[i];
-    const W t = block_sum(s);
+    const W t = reduce_partials(partial, m);
     if (threadIdx.x == 0) {
         *out = t;
         if (!(t > W(0))) {
@@ -453,12 +496,12 @@ static int cg_loop(smax_hip_ctx ctx, const Mat &A, const V *val, const float *di
             hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG>), sg, dim3(kCgThreads), 0,
                                ctx->stream, n, A.C, A.chunk_ptr, A.chunk_len, A.col,
                                val, diag, p, q, partial, ctl);
-            hipLaunchKernelGGL((k_reduce_pq<W>), dim3(1), dim3(kCgThreads), 0,
+            hipLaunchKernelGGL((k_reduce_pq<W>), dim3(1), dim3(kReduceThreads), 0,
                                ctx->stream, partial, static_cast<int64_t>(sg.x),
                                scal + 2, ctl);
             hipLaunchKernelGGL((k_update_xr<W>), ug, dim3(kCgThreads), 0, ctx->stream,
                                n, scal, cur, 2, x, r, p, q, partial, ctl);
-            hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kCgThreads), 0,
+            hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kReduceThreads), 0,
                                ctx->stream, partial, static_cast<int64_t>(ug.x),
                                scal + nxt, ctl, stop2);
             hipLaunchKernelGGL((k_update_p<W>), ug, dim3(kCgThreads), 0, ctx->stream,
@@ -489,7 +532,7 @@ static int residual(smax_hip_ctx ctx, const Mat &A, const double *val,
     const dim3 sg = grid_for(A.n, kCgThreads);
     hipLaunchKernelGGL(k_residual, sg, dim3(kCgThreads), 0, ctx->stream, A.n, A.C,
                        A.chunk_ptr, A.chunk_len, A.col, val, b, x, r, partial);
-    hipLaunchKernelGGL((k_reduce<double>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+    hipLaunchKernelGGL((k_reduce<double>), dim3(1), dim3(kReduceThreads), 0, ctx->stream,
                        partial, static_cast<int64_t>(sg.x), scal + idx, nullptr, -1.0);
     SMAX_HIP_LAUNCHED(ctx, "k_residual");
     return read_words(ctx, scal + idx, rr, 1);
@@ -501,7 +544,7 @@ static int norm2(smax_hip_ctx ctx, int64_t n, const W *v, W *tmp, W *partial, W 
                  int idx, W *out) {
     hipLaunchKernelGGL((k_copy_dot<W>), upd_grid(n), dim3(kCgThreads), 0, ctx->stream,
                        n, v, tmp, partial);
-    hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+    hipLaunchKernelGGL((k_reduce<W>), dim3(1), dim3(kReduceThreads), 0, ctx->stream,
                        partial, static_cast<int64_t>(upd_grid(n).x), scal + idx,
                        nullptr, W(-1));
     SMAX_HIP_LAUNCHED(ctx, "norm2");
@@ -592,7 +635,7 @@ static int reliable_cg(smax_hip_ctx ctx, const Mat &M, const double *val64,
     if (bnorm == 0 || std::sqrt(rr64) <= target) return SMAX_HIP_SUCCESS;
     hipLaunchKernelGGL(k_start_reliable, ug, dim3(kCgThreads), 0, ctx->stream, n, r64,
                        r, p, xl, partial32);
-    hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+    hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kReduceThreads), 0, ctx->stream,
                        partial32, static_cast<int64_t>(ug.x), scal32 + 0, nullptr, -1.0f);
     hipLaunchKernelGGL(k_zero_ctl, dim3(1), dim3(64), 0, ctx->stream, ctl);
     SMAX_HIP_LAUNCHED(ctx, "reliable start");
@@ -611,13 +654,13 @@ static int reliable_cg(smax_hip_ctx ctx, const Mat &M, const double *val64,
                 hipLaunchKernelGGL((k_spmv_dot<V, float, DIAG>), sg, dim3(kCgThreads),
                                    0, ctx->stream, n, M.C, M.chunk_ptr, M.chunk_len,
                                    M.col, val, diag, p, q, partial32, ctl);
-                hipLaunchKernelGGL((k_reduce_pq<float>), dim3(1), dim3(kCgThreads), 0,
+                hipLaunchKernelGGL((k_reduce_pq<float>), dim3(1), dim3(kReduceThreads), 0,
                                    ctx->stream, partial32, static_cast<int64_t>(sg.x),
                                    scal32 + 2, ctl);
                 hipLaunchKernelGGL((k_update_xr<float>), ug, dim3(kCgThreads), 0,
                                    ctx->stream, n, scal32, cur, 2, xl, r, p, q,
                                    partial32, ctl);
-                hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0,
+                hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kReduceThreads), 0,
                                    ctx->stream, partial32, static_cast<int64_t>(ug.x),
                                    scal32 + nxt, ctl, trig2);
                 hipLaunchKernelGGL((k_update_p<float>), ug, dim3(kCgThreads), 0,
@@ -670,7 +713,7 @@ static int reliable_cg(smax_hip_ctx ctx, const Mat &M, const double *val64,
         else
             hipLaunchKernelGGL(k_start_reliable, ug, dim3(kCgThreads), 0, ctx->stream,
                                n, r64, r, p, xl, partial32);
-        hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+        hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kReduceThreads), 0, ctx->stream,
                            partial32, static_cast<int64_t>(ug.x), scal32 + par, nullptr,
                            -1.0f);
         hipLaunchKernelGGL(k_clear_done, dim3(1), dim3(64), 0, ctx->stream, ctl);
@@ -842,7 +885,7 @@ int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
         ++outer;
         hipLaunchKernelGGL(k_start_inner, upd_grid(n), dim3(kCgThreads), 0,
                            ctx->stream, n, scal64, 0, r64, r, p, d, partial32);
-        hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kCgThreads), 0, ctx->stream,
+        hipLaunchKernelGGL((k_reduce<float>), dim3(1), dim3(kReduceThreads), 0, ctx->stream,
                            partial32, static_cast<int64_t>(upd_grid(n).x), scal32 + 0,
                            nullptr, -1.0f);
         SMAX_HIP_LAUNCHED(ctx, "k_start_inner");
