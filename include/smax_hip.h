@@ -7,7 +7,9 @@
  * (the other SMAX::IntType values are rejected with SMAX_HIP_ERROR_UNSUPPORTED
  * by smax_hip_check_types).  Every compute entry point enqueues work on the
  * context's current stream and returns without synchronising; a non-zero
- * status means nothing was enqueued, and smax_hip_last_error() describes it.
+ * status from one that launches a single kernel means nothing was enqueued,
+ * and smax_hip_last_error() describes the failure.  Conversions, plans and
+ * solvers allocate and synchronise.
  *
  * Each entry point names the reference interface it replaces (paths relative
  * to the SmaxKernels source tree).  The reference CUDA platform does
@@ -214,7 +216,9 @@ int smax_hip_scs_spmv_plan(smax_hip_ctx ctx, int value_type, smax_hip_scs_plan p
 /*
  * Y = A X on a SELL matrix, X (n_cols x k) and Y (n_rows x k) row-major, Y
  * in original row order through slot_row.  Each output is a sequential fma
- * chain over the row's SELL elements.  No reference counterpart.
+ * chain over the row's padded SELL elements (a padding element adds
+ * 0 * X[0, v]).  No reference counterpart; see smax_hip_scs_apply_spmm for
+ * the chain over stored entries only.
  */
 int smax_hip_scs_spmm(smax_hip_ctx ctx, int value_type, int64_t C,
                       int64_t n_chunks, int64_t n_rows,
@@ -269,8 +273,10 @@ int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs);
  * entries) or smax_hip_scs_spmv_unpermuted (unpermuted = 1).  It uses the
  * handle's slot lengths so each lane stops at its row's last stored element
  * (padding is neither read nor multiplied; a non-finite x[0] still reaches
- * every padded row as the reference's 0*x[0] term would), and picks the
- * workgroup-to-XCD schedule from sigma.  No reference counterpart.
+ * every padded row as the reference's 0*x[0] term would), and runs in the
+ * handle's execution schedule: long chunks first, contiguous chunk ranges per
+ * XCD when sigma*16 <= n_rows_padded, and a capped occupancy for layouts with
+ * little padding.  No reference counterpart.
  */
 int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                        const void *x, void *y, int unpermuted);
