@@ -195,6 +195,29 @@ def cg_section(ctx) -> dict:
     return out
 
 
+def lowp_section(ctx, A, rp, col, steps, warmup) -> dict:
+    """The 4M-row Lomax matrix with FP32 and BF16 values (FP32 x, y and
+    accumulation), SELL-64 at sigma = 4096 and 65536 through the handle path."""
+    out = {}
+    for name, vt, vals, vb in (("f32", capi.F32, A.val.astype(np.float32), 4),
+                               ("bf16", capi.BF16, matgen.to_bf16(A.val), 2)):
+        ba = A.nnz * (vb + 4) + (A.n_rows + 1) * 4 + A.n_cols * 4 + A.n_rows * 4
+        dv = ctx.upload(vals)
+        dx = ctx.upload(matgen.uniform_f32(matgen.SEED_X, A.n_cols))
+        for sigma in (4096, 65536):
+            s = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, dv, vb, 64, sigma)
+            yy = ctx.zeros(s.info.n_rows_padded, np.float32)
+            ms = time_launches(ctx, lambda: s.apply(vt, dx, yy), steps, warmup)
+            out[f"{name}_C64_s{sigma}"] = {"ms": round(ms, 4), "B_alg": ba,
+                                           "GBs_alg": round(ba / ms / 1e6, 1)}
+            yy.free()
+            s.destroy()
+        dv.free()
+        dx.free()
+    log(f"SELL low precision {out}")
+    return out
+
+
 def c4_section(ctx, steps, warmup) -> dict:
     """FP32 SpMM with k = 32 row-major columns on the 1M-row Lomax band matrix
     (nnz ~ 16M): forward Y = A X (CRS and SELL-64-N), adjoint A^T W through
@@ -280,7 +303,7 @@ def main() -> int:
     ap.add_argument("--cs", default="64,32", help="comma list of SELL chunk sizes")
     ap.add_argument("--only", default="",
                     help="comma list of sections to run: "
-                         "stream,sell,crs,c1,cg,c4,c5,cpu")
+                         "stream,sell,lowp,crs,c1,cg,c4,c5,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -338,6 +361,9 @@ def main() -> int:
                 yy.free()
                 s.destroy()
         extra["sell_f64"] = sell
+
+    if want("lowp"):
+        extra["sell_lowp"] = lowp_section(ctx, A, rp, col, args.steps, args.warmup)
 
     crs = {}
     if want("crs"):
