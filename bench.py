@@ -10,7 +10,8 @@ GB/s: B_alg = nnz*(8+4) + (N+1)*4 + N*8 (x) + N*8 (y) per SpMV, divided by
 the average kernel time measured with HIP events on the library's stream.
 
 Also reported: every (C, sigma) variant, CRS (lane-per-row and LDS row-block
-schedules), the 512^2 5-point Laplacian (warm and cold L2/MALL), FP64 CG vs
+schedules), rocSPARSE's CSR SpMV algorithms on the same arrays (library
+baseline, comparison only), FP32/BF16-value SELL, the 512^2 5-point Laplacian (warm and cold L2/MALL), FP64 CG vs
 BF16/FP32-value refined CG on the 160^3 27-point stencil, FP32 SpMM k=32 with
 its adjoint and SDDMM (1M-row Lomax band), BCRS 4x4 BF16 SpMM forward and
 adjoint (96^3 7-point block adjacency, k = 8 and 64), the device copy
@@ -195,6 +196,43 @@ def cg_section(ctx) -> dict:
     return out
 
 
+def vendor_section(ctx, A, rp, col, val, dx, steps, warmup) -> dict:
+    """rocSPARSE CSR SpMV (FP64) on the same device arrays: a library
+    baseline for comparison only (smaxkernels_amd/vendor/rocsparse_cmp.cpp)."""
+    import ctypes as C
+    path = ROOT / "smaxkernels_amd" / "_build" / "librocsparse_cmp.so"
+    try:
+        lib = C.CDLL(str(path))
+    except OSError as e:
+        return {"error": f"rocSPARSE comparison not loaded: {e}"}
+    lib.rocsparse_cmp_create.restype = C.c_void_p
+    lib.rocsparse_cmp_create.argtypes = [C.c_int] + [C.c_int64] * 3 + [C.c_void_p] * 6
+    lib.rocsparse_cmp_spmv.argtypes = [C.c_void_p]
+    lib.rocsparse_cmp_destroy.argtypes = [C.c_void_p]
+    stream = ctx.lib.smax_hip_get_stream(ctx.handle)
+    y = ctx.zeros(A.n_rows, np.float64)
+    ctx.crs_spmv(capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, val, dx, y, None)
+    ours = ctx.download(y)
+    balg = b_alg(A.n_rows, A.n_cols, A.nnz)
+    out = {}
+    for alg, name in ((1, "csr_adaptive"), (2, "csr_rowsplit"), (3, "csr_lrb"),
+                      (4, "csr_nnzsplit")):
+        h = lib.rocsparse_cmp_create(alg, A.n_rows, A.n_cols, A.nnz, rp.ptr, col.ptr,
+                                     val.ptr, dx.ptr, y.ptr, stream)
+        if not h:
+            out[name] = None
+            continue
+        ms = time_launches(ctx, lambda: lib.rocsparse_cmp_spmv(h), steps, warmup)
+        got = ctx.download(y)
+        out[name] = {"ms": round(ms, 4), "GBs_alg": round(balg / ms / 1e6, 1),
+                     "max_rel_diff_vs_reference_chain":
+                         float(np.max(np.abs(got - ours)) / np.max(np.abs(ours)))}
+        lib.rocsparse_cmp_destroy(h)
+    y.free()
+    log(f"rocSPARSE {out}")
+    return out
+
+
 def lowp_section(ctx, A, rp, col, steps, warmup) -> dict:
     """The 4M-row Lomax matrix with FP32 and BF16 values (FP32 x, y and
     accumulation), SELL-64 at sigma = 4096 and 65536 through the handle path."""
@@ -303,7 +341,7 @@ def main() -> int:
     ap.add_argument("--cs", default="64,32", help="comma list of SELL chunk sizes")
     ap.add_argument("--only", default="",
                     help="comma list of sections to run: "
-                         "stream,sell,lowp,crs,c1,cg,c4,c5,cpu")
+                         "stream,sell,vendor,lowp,crs,c1,cg,c4,c5,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -361,6 +399,10 @@ def main() -> int:
                 yy.free()
                 s.destroy()
         extra["sell_f64"] = sell
+
+    if want("vendor"):
+        extra["rocsparse_csr_f64"] = vendor_section(ctx, A, rp, col, val, dx, args.steps,
+                                                    args.warmup)
 
     if want("lowp"):
         extra["sell_lowp"] = lowp_section(ctx, A, rp, col, args.steps, args.warmup)

@@ -106,10 +106,30 @@ def build_interface() -> Path:
     return lib
 
 
+def build_vendor_cmp() -> Path | None:
+    """rocSPARSE CSR SpMV wrapper used by bench.py as a library baseline
+    (comparison only); skipped when rocSPARSE is not installed."""
+    rocm = Path(HIPCC).resolve().parent.parent
+    if not (rocm / "lib" / "librocsparse.so").exists():
+        return None
+    OUT.mkdir(exist_ok=True)
+    lib = OUT / "librocsparse_cmp.so"
+    tmp = OUT / "librocsparse_cmp.so.tmp"
+    cxx = shutil.which("g++") or "c++"
+    _run([cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
+          "-Wno-deprecated-declarations", f"-I{rocm / 'include'}",
+          str(PKG / "vendor" / "rocsparse_cmp.cpp"), "-o", str(tmp),
+          f"-L{rocm / 'lib'}", "-lrocsparse", "-lamdhip64",
+          f"-Wl,-rpath,{rocm / 'lib'}"])
+    os.replace(tmp, lib)
+    return lib
+
+
 def build_all() -> None:
     build_matgen()
     build_hip()
     build_interface()
+    build_vendor_cmp()
 
 
 if __name__ == "__main__":
