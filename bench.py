@@ -196,6 +196,35 @@ def cg_section(ctx) -> dict:
     return out
 
 
+def mtx_section(ctx, path, steps, warmup) -> dict:
+    """FP64 SpMV on a Matrix Market file, read as the reference's benchmarks
+    read it (matgen.read_mtx); x uniform [-1, 1) from the benchmark seed."""
+    M = matgen.read_mtx(path)
+    log(f"{path}: {M.n_rows} x {M.n_cols}, nnz {M.nnz}")
+    x = matgen.uniform_f64(matgen.SEED_X, M.n_cols)
+    ba = b_alg(M.n_rows, M.n_cols, M.nnz)
+    rp, col, val, dx = ctx.upload(M.row_ptr), ctx.upload(M.col), ctx.upload(M.val), ctx.upload(x)
+    y = ctx.zeros(M.n_rows, np.float64)
+    out = {"file": str(path), "n_rows": M.n_rows, "nnz": M.nnz, "B_alg": ba}
+    plan = ctx.crs_plan(M.n_rows, M.nnz, rp)
+    ms = time_launches(ctx, lambda: ctx.crs_spmv(capi.F64, M.n_rows, M.n_cols, M.nnz,
+                                                 rp, col, val, dx, y, plan), steps, warmup)
+    out["crs_row_blocks"] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1)}
+    ctx.crs_plan_destroy(plan)
+    for sigma in (1, 4096):
+        s = ctx.scs_from_crs(M.n_rows, M.n_cols, M.nnz, rp, col, val, 8, 64, sigma)
+        yy = ctx.zeros(s.info.n_rows_padded, np.float64)
+        ms = time_launches(ctx, lambda: s.apply(capi.F64, dx, yy), steps, warmup)
+        out[f"sell64_s{sigma}"] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1),
+                                   "fill": round(s.info.n_elements / max(M.nnz, 1), 4)}
+        yy.free()
+        s.destroy()
+    for d in (rp, col, val, dx, y):
+        d.free()
+    log(f"mtx {out}")
+    return out
+
+
 def vendor_section(ctx, A, rp, col, val, dx, steps, warmup) -> dict:
     """rocSPARSE CSR SpMV (FP64) on the same device arrays: a library
     baseline for comparison only (smaxkernels_amd/vendor/rocsparse_cmp.cpp)."""
@@ -339,6 +368,9 @@ def main() -> int:
     ap.add_argument("--sigmas", default="4096,65536,N",
                     help="comma list of SELL sigma values (N = n_rows)")
     ap.add_argument("--cs", default="64,32", help="comma list of SELL chunk sizes")
+    ap.add_argument("--mtx", default="",
+                    help="also time FP64 SpMV on this Matrix Market file (CRS row "
+                         "blocks and SELL-64 at sigma 1 and 4096), reported under extra")
     ap.add_argument("--only", default="",
                     help="comma list of sections to run: "
                          "stream,sell,vendor,lowp,crs,c1,cg,c4,c5,cpu")
@@ -457,6 +489,9 @@ def main() -> int:
             d.free()
         ps.destroy()
         log(f"C1 {extra['c1_poisson512_f64_sell64_1']}")
+
+    if args.mtx:
+        extra["mtx"] = mtx_section(ctx, args.mtx, args.steps, args.warmup)
 
     if want("cg"):
         extra["c3_cg"] = cg_section(ctx)

@@ -84,8 +84,8 @@ def build_matgen() -> Path:
     lib = OUT / "libsmax_matgen.so"
     tmp = OUT / "libsmax_matgen.so.tmp"
     cc = shutil.which("gcc") or "cc"
-    _run([cc, *HOST_CFLAGS, "-shared", str(GEN / "matgen.c"), "-o", str(tmp),
-          "-lm"])
+    _run([cc, *HOST_CFLAGS, "-shared", str(GEN / "matgen.c"), str(GEN / "mtx.c"),
+          "-o", str(tmp), "-lm"])
     os.replace(tmp, lib)
     return lib
 

@@ -46,6 +46,8 @@ def _load():
             "smax_gen_block7": ([i64, i64, u64, d, vp, vp, vp], None),
             "smax_gen_f64_to_bf16": ([i64, vp, vp], None),
             "smax_gen_f64_to_f32": ([i64, vp, vp], None),
+            "smax_mtx_header": ([C.c_char_p, vp, vp, vp, vp], C.c_int),
+            "smax_mtx_read": ([C.c_char_p, i64, vp, vp, vp], i64),
         }
         for n, (a, r) in sig.items():
             f = getattr(lib, n)
@@ -138,3 +140,33 @@ def block7(n: int, b: int = 4, seed: int = SEED_VALUES, shift: float = 8.0) -> C
     _load().smax_gen_block7(n, b, seed, shift, rp.ctypes.data, col.ctypes.data,
                             val.ctypes.data)
     return CRS(N, N, rp, col, val)
+
+
+MTX_ERRORS = {-1: "cannot open file", -2: "missing or malformed Matrix Market banner",
+              -3: "unsupported Matrix Market type (coordinate real/integer/pattern, "
+                  "general/symmetric only)",
+              -4: "malformed size line", -5: "malformed or out-of-range entry",
+              -6: "capacity", -7: "matrix does not fit int32 CRS", -8: "out of memory"}
+
+
+def read_mtx(path) -> CRS:
+    """CRS of a Matrix Market coordinate file, built as the reference's
+    benchmarks build it (COOMatrix::read_from_mtx + convert_coo_to_crs,
+    examples/matrix_structs.hpp:52-180, 338-388): 0-based indices, pattern
+    values 0.01, symmetric entries mirrored, rows stable-sorted (file order
+    within a row)."""
+    b = str(path).encode()
+    n_rows, n_cols, n_ent = C.c_int64(), C.c_int64(), C.c_int64()
+    flags = C.c_int()
+    rc = _load().smax_mtx_header(b, C.byref(n_rows), C.byref(n_cols), C.byref(n_ent),
+                                 C.byref(flags))
+    if rc:
+        raise ValueError(f"{path}: {MTX_ERRORS.get(rc, rc)}")
+    cap = n_ent.value * (2 if flags.value & 1 else 1)
+    rp = np.empty(n_rows.value + 1, np.int32)
+    col = np.empty(max(cap, 1), np.int32)
+    val = np.empty(max(cap, 1), np.float64)
+    nnz = _load().smax_mtx_read(b, cap, rp.ctypes.data, col.ctypes.data, val.ctypes.data)
+    if nnz < 0:
+        raise ValueError(f"{path}: {MTX_ERRORS.get(nnz, nnz)}")
+    return CRS(n_rows.value, n_cols.value, rp, col[:nnz].copy(), val[:nnz].copy())
