@@ -113,6 +113,32 @@ def stream_peak(ctx, steps=10) -> dict:
             "array_GiB": 4}
 
 
+def gather_ceiling(ctx, A, steps, warmup) -> dict:
+    """Random-gather ceiling for A's x accesses: n_rows lanes each read 16
+    uniformly random entries of x from a window as wide as A's band
+    (max - min of col - row), with no index stream.  An SpMV over A makes
+    the same number of gathers over the same window and cannot beat this
+    rate; its own gathers per second over this ceiling is the fraction of
+    the gather roofline it reaches."""
+    rows = np.repeat(np.arange(A.n_rows, dtype=np.int64), np.diff(A.row_ptr))
+    rel = A.col.astype(np.int64) - rows
+    n = min(A.n_rows, A.n_cols)
+    window = int(min(rel.max() - rel.min() + 1, n))
+    del rows, rel
+    per_lane = 16
+    x = ctx.upload(np.ones(n, np.float64))
+    y = ctx.zeros(n, np.float64)
+    ms = time_launches(ctx, lambda: ctx.gather_rate(x, y, n, window, per_lane), steps, warmup)
+    ok = bool(np.all(ctx.download(y) == per_lane))
+    x.free()
+    y.free()
+    if not ok:
+        raise RuntimeError("gather_rate kernel returned wrong sums")
+    return {"window_entries": window, "window_MB": round(window * 8 / 1e6, 2),
+            "lanes": n, "per_lane": per_lane, "ms": round(ms, 4),
+            "G_gathers_per_s": round(n * per_lane / ms / 1e6, 1)}
+
+
 def hbm_traffic(kernel: str, n_rows: int, fmt: str):
     """Bytes per launch of `kernel` between L2 and the fabric (HBM + MALL) on
     the N-row Lomax workload in format `fmt`, from the committed rocprofv3
@@ -373,7 +399,7 @@ def main() -> int:
                          "blocks and SELL-64 at sigma 1 and 4096), reported under extra")
     ap.add_argument("--only", default="",
                     help="comma list of sections to run: "
-                         "stream,sell,vendor,lowp,crs,c1,cg,c4,c5,cpu")
+                         "stream,sell,gather,vendor,lowp,crs,c1,cg,c4,c5,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -431,6 +457,12 @@ def main() -> int:
                 yy.free()
                 s.destroy()
         extra["sell_f64"] = sell
+
+    gather = None
+    if want("gather"):
+        log("random-gather ceiling over the band window")
+        gather = gather_ceiling(ctx, A, 50, 5)
+        extra["gather_ceiling"] = gather
 
     if want("vendor"):
         extra["rocsparse_csr_f64"] = vendor_section(ctx, A, rp, col, val, dx, args.steps,
@@ -519,6 +551,12 @@ def main() -> int:
     if peak_gbs:
         roof["frac_of_measured_stream"] = round(value / peak_gbs, 4)
         roof["measured_stream_GBs"] = peak_gbs
+    if gather:
+        rate = A.nnz / ms / 1e6
+        roof["gather"] = {"spmv_G_gathers_per_s": round(rate, 1),
+                          "ceiling_G_gathers_per_s": gather["G_gathers_per_s"],
+                          "frac": round(rate / gather["G_gathers_per_s"], 4),
+                          "window_MB": gather["window_MB"]}
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GB/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

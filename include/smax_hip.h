@@ -452,6 +452,14 @@ int smax_hip_stream_copy(smax_hip_ctx ctx, const double *src, double *dst,
                          int64_t n);
 int smax_hip_stream_triad(smax_hip_ctx ctx, double *a, const double *b,
                           const double *c, double s, int64_t n);
+/* Random-gather reference for SpMVs whose x accesses are scattered: lane i
+ * (i < n) reads per_lane (8, 16 or 32) entries of x[n] at pseudo-random
+ * positions of a window of `window` entries centred on i (shifted to stay
+ * inside [0, n)) and writes their sum to y[i].  No index array is read, so
+ * n * per_lane / time is the gather ceiling for that window.  No reference
+ * counterpart (a measurement helper, like the two above). */
+int smax_hip_gather_rate(smax_hip_ctx ctx, const double *x, double *y, int64_t n,
+                         int64_t window, int per_lane);
 
 #ifdef __cplusplus
 }

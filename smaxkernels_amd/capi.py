@@ -126,6 +126,7 @@ _SIGS = {
                        C.POINTER(CgReport)], C.c_int),
     "smax_hip_stream_copy": ([vp, vp, vp, i64], C.c_int),
     "smax_hip_stream_triad": ([vp, vp, vp, vp, C.c_double, i64], C.c_int),
+    "smax_hip_gather_rate": ([vp, vp, vp, i64, i64, C.c_int], C.c_int),
 }
 
 _lib = None
@@ -366,6 +367,10 @@ class Context:
     def stream_triad(self, a, b, c, s, n) -> None:
         self._check(self.lib.smax_hip_stream_triad(self.handle, _ptr(a),
                                                    _ptr(b), _ptr(c), s, n))
+
+    def gather_rate(self, x, y, n, window, per_lane) -> None:
+        self._check(self.lib.smax_hip_gather_rate(self.handle, _ptr(x), _ptr(y), n,
+                                                  window, per_lane))
 
     def cg(self, A: "Scs", b, x, params: CgParams, mixed: bool) -> CgReport:
         rep = CgReport()

@@ -123,6 +123,40 @@ __global__ void triad_kernel(double *__restrict__ a,
     if (i == 0 && (n & 1)) a[n - 1] = __builtin_fma(s, c[n - 1], b[n - 1]);
 }
 
+// Random-gather rate: lane i reads K entries of x at pseudo-random positions
+// of the band [i - window/2, i - window/2 + window) (shifted inside [0, n)),
+// all K requested before any is used, as an SpMV lane gathers its row's x
+// entries.  No index stream is read, so the time is the gather path alone.
+__device__ __forceinline__ uint32_t gather_mix(uint32_t a) {
+    a ^= a >> 16;
+    a *= 0x7feb352dU;
+    a ^= a >> 15;
+    a *= 0x846ca68bU;
+    a ^= a >> 16;
+    return a;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ x,
+                                                     double *__restrict__ y, int64_t n,
+                                                     int64_t window) {
+    const int64_t i = smaxhip::xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    int64_t lo = i - window / 2;
+    lo = lo < 0 ? 0 : (lo > n - window ? n - window : lo);
+    uint32_t h = gather_mix(static_cast<uint32_t>(i) * 2654435761U + 12345U);
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        h = gather_mix(h + static_cast<uint32_t>(k));
+        v[k] = x[lo + static_cast<int64_t>(h % static_cast<uint32_t>(window))];
+    }
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc += v[k];
+    y[i] = acc;
+}
+
 // Persistent-style grid: enough blocks to fill 256 CUs several times over.
 static dim3 stream_grid(int64_t n2) {
     int64_t blocks = ceil_div(n2, 256);
@@ -312,6 +346,29 @@ int smax_hip_stream_triad(smax_hip_ctx ctx, double *a, const double *b,
     hipLaunchKernelGGL(triad_kernel, stream_grid(n / 2), dim3(256), 0,
                        ctx->stream, a, b, c, s, n);
     SMAX_HIP_LAUNCHED(ctx, "triad_kernel");
+    return SMAX_HIP_SUCCESS;
+}
+
+int smax_hip_gather_rate(smax_hip_ctx ctx, const double *x, double *y, int64_t n,
+                         int64_t window, int per_lane) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, x && y && n > 0 && n <= INT32_MAX && window > 0 && window <= n,
+                     "smax_hip_gather_rate: need 0 < window <= n <= 2^31-1");
+    SMAX_HIP_REQUIRE(ctx, per_lane == 8 || per_lane == 16 || per_lane == 32,
+                     "smax_hip_gather_rate: per_lane must be 8, 16 or 32");
+    const dim3 g = grid_for(n, 256);
+    switch (per_lane) {
+    case 8:
+        hipLaunchKernelGGL(gather_kernel<8>, g, dim3(256), 0, ctx->stream, x, y, n, window);
+        break;
+    case 16:
+        hipLaunchKernelGGL(gather_kernel<16>, g, dim3(256), 0, ctx->stream, x, y, n, window);
+        break;
+    default:
+        hipLaunchKernelGGL(gather_kernel<32>, g, dim3(256), 0, ctx->stream, x, y, n, window);
+        break;
+    }
+    SMAX_HIP_LAUNCHED(ctx, "gather_kernel");
     return SMAX_HIP_SUCCESS;
 }
 
