@@ -382,8 +382,12 @@ int smax_hip_scs_transpose_from_crs(smax_hip_ctx ctx, int64_t n_rows,
  * X[col[e]*k+v] for every stored (non-padding) element e of row r, i.e. the
  * gradient of sum(Y o W) w.r.t. A's CRS values when G = W and Y = A X.
  * G is n_rows x k, X is n_cols x k, both row-major, original row order.
- * Accumulates in the value type's accumulation precision with a fixed
- * pairwise tree over v.  No reference counterpart.
+ * Accumulates in the value type's accumulation precision with a tree that
+ * depends only on k: g = clamp(P/4, 1, 64) partial sums (P the smallest
+ * power of two >= k), partial l an fma chain over v in [l*q, min(k, l*q+q)),
+ * q = ceil(k/g), then a butterfly over the g partials.  G and X aligned to
+ * 16 bytes take a faster kernel with identical results.  No reference
+ * counterpart.
  */
 int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                        int64_t k, const void *G, const void *X, void *dvals);

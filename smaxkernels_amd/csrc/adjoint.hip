@@ -10,12 +10,15 @@
 // pointing back at A's CRS nonzeros.
 //
 // SDDMM: dvals[src_index[e]] = sum_v G[r*k+v] * X[col[e]*k+v] for every stored
-// element e of SELL row r; a group of up to 64 lanes takes one element, each
-// lane a strided part of v (fma chain), then a fixed xor-butterfly sums the
-// group: the reduction tree depends only on k.  For k a power of two up to
-// 64 the lanes of a group share one row (SpMM order), so the G row is read
-// once per row; other k take one group per stored element, with the row of
-// each element from a per-element row array built on first use.
+// element e of SELL row r.  The reduction tree depends only on k: with P the
+// smallest power of two >= k, g = clamp(P / 4, 1, 64) lanes share an element,
+// lane l sums v in [l*Q, min(k, l*Q + Q)), Q = ceil(k / g), with an fma chain
+// from zero, and an xor-butterfly adds the g partial sums.  For k a power of
+// two up to 64 the lanes of a group share one row (SpMM order): the G row is
+// read once per row and each element costs one coalesced X row with 16-byte
+// loads.  Other k take one group per stored element, with the row of each
+// element from a per-element row array built on first use.  Both kernels
+// evaluate the same tree, so dvals do not depend on which one runs.
 #include <utility>
 
 #include "common.hpp"
@@ -42,7 +45,24 @@ __global__ __launch_bounds__(256) void k_elem_row(
     for (int j = 0; j < clen; ++j) elem_row[base + static_cast<int64_t>(j) * C] = j < len ? r : -1;
 }
 
-// One group of G lanes per stored element.
+// Lanes per element and per-lane run length of the SDDMM tree for k.
+__host__ __device__ constexpr int64_t sddmm_group(int64_t k) {
+    int64_t p = 1;
+    while (p < k) p <<= 1;
+    return p / 4 < 1 ? 1 : (p / 4 > 64 ? 64 : p / 4);
+}
+__host__ __device__ constexpr int64_t sddmm_run(int64_t k) {
+    return (k + sddmm_group(k) - 1) / sddmm_group(k);
+}
+
+// The xor-butterfly over G lanes: every lane ends with the group's sum.
+template <int G, typename A> __device__ __forceinline__ A group_sum(A acc) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
+    return acc;
+}
+
+// One group of G = sddmm_group(k) lanes per stored element.
 template <int VT, int G>
 __global__ __launch_bounds__(256) void scs_sddmm_kernel(
     int64_t n_elements, int64_t k, const int32_t *__restrict__ col,
@@ -61,19 +81,51 @@ __global__ __launch_bounds__(256) void scs_sddmm_kernel(
     if (s >= 0) {
         const int64_t r = elem_row[e];
         const int64_t c = col[e];
-        for (int64_t v = lane; v < k; v += G)
-            acc = T::fma_(Gm[r * k + v], X[c * k + v], acc);
+        const int64_t q = sddmm_run(k);
+        const int64_t v1 = (lane + 1) * q < k ? (lane + 1) * q : k;
+        for (int64_t v = lane * q; v < v1; ++v) acc = T::fma_(Gm[r * k + v], X[c * k + v], acc);
     }
-#pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
+    acc = group_sum<G>(acc);
     if (s >= 0 && lane == 0) dvals[s] = acc;
 }
 
-// k a power of two <= 64: k lanes per slot (row) in SpMM order, long
-// chunks first.  Lane v keeps G[r, v] in a register for the whole row and
-// each element costs one coalesced X row; per element the arithmetic is the
-// kernel above's (one fma per lane, then the xor-butterfly), so dvals are
-// bit-identical to it.
+// Q consecutive entries from p; with Q * sizeof(A) a multiple of 16 the host
+// has checked that p is 16-byte aligned, and they come as 16-byte loads.
+template <typename A, int Q>
+__device__ __forceinline__ void load_run(const A *__restrict__ p, A (&out)[Q]) {
+    if constexpr ((Q * sizeof(A)) % 16 == 0) {
+        constexpr int W = 16 / sizeof(A);
+        typedef A V __attribute__((ext_vector_type(W)));
+        const V *q = reinterpret_cast<const V *>(p);
+#pragma unroll
+        for (int h = 0; h < Q / W; ++h) {
+            const V w = q[h];
+#pragma unroll
+            for (int i = 0; i < W; ++i) out[h * W + i] = w[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) out[i] = p[i];
+    }
+}
+
+// k = K a power of two <= 64: GL = sddmm_group(K) lanes per slot (row) in
+// SpMM order, long chunks first.  Lane l keeps G[r, l*Q .. l*Q+Q) in
+// registers for the whole row; each element costs one coalesced X row read
+// with 16-byte loads, Q fmas per lane and the butterfly over GL lanes.  The
+// chain over the row's elements is software-pipelined (the next U column and
+// src indices are loaded before this group's X rows), and the U results of a
+// group, consecutive CRS positions of one row, leave in one store
+// instruction from U lanes (GL / U instructions when GL < U).
+template <typename A, int U>
+__device__ __forceinline__ A pick(const A (&r)[U], int idx) {
+    A out = r[0];
+#pragma unroll
+    for (int u = 1; u < U; ++u)
+        if (idx == u) out = r[u];
+    return out;
+}
+
 template <int VT, int K>
 __global__ __launch_bounds__(256) void scs_sddmm_rows(
     int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
@@ -85,7 +137,11 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
     typename vt<VT>::vec_t *__restrict__ dvals) {
     using T = vt<VT>;
     using A = typename T::vec_t;
-    const int64_t ck = C * K;
+    constexpr int GL = static_cast<int>(sddmm_group(K));
+    constexpr int Q = K / GL;
+    constexpr int U = 8;
+    static_assert(GL * Q == K, "K must be a power of two");
+    const int64_t ck = C * GL;
     int64_t pos, li;
     if (cpb > 1) {
         if (threadIdx.x >= cpb * ck) return;
@@ -97,41 +153,68 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
     }
     if (pos >= n_chunks || li >= ck) return;
     const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
-    const int64_t i = li / K;
-    const int v = static_cast<int>(li % K);
+    const int64_t i = li / GL;
+    const int l = static_cast<int>(li % GL);
     const int64_t slot = chunk * C + i;
     const int32_t r = slot_row[slot];
     if (r >= n_rows) return;
     const int len = slot_len[slot];
     const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
-    const A g = Gm[static_cast<int64_t>(r) * K + v];
-    constexpr int U = 4;
+    A g[Q];
+    load_run<A, Q>(Gm + static_cast<int64_t>(r) * K + l * Q, g);
     int j = 0;
-    for (; j + U <= len; j += U) {
-        int32_t c[U], sv[U];
+    if (len >= U) {
+        int32_t c0[U], s0[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t e = base + static_cast<int64_t>(j + u) * C;
-            c[u] = col[e];
-            sv[u] = src[e];
+            c0[u] = col[base + u * C];
+            s0[u] = src[base + u * C];
         }
-        A xv[U];
+        for (;;) {
+            const bool more = j + 2 * U <= len;
+            int32_t c1[U], s1[U];
+            if (more) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = X[static_cast<int64_t>(c[u]) * K + v];
+                for (int u = 0; u < U; ++u) {
+                    c1[u] = col[base + static_cast<int64_t>(j + U + u) * C];
+                    s1[u] = src[base + static_cast<int64_t>(j + U + u) * C];
+                }
+            }
+            A xv[U][Q];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            A acc = T::fma_(g, xv[u], A(0));
+            for (int u = 0; u < U; ++u)
+                load_run<A, Q>(X + static_cast<int64_t>(c0[u]) * K + l * Q, xv[u]);
+            A res[U];
 #pragma unroll
-            for (int off = K / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, K);
-            if (v == 0) dvals[sv[u]] = acc;
+            for (int u = 0; u < U; ++u) {
+                A acc = 0;
+#pragma unroll
+                for (int t = 0; t < Q; ++t) acc = T::fma_(g[t], xv[u][t], acc);
+                res[u] = group_sum<GL>(acc);
+            }
+#pragma unroll
+            for (int w = 0; w < U; w += GL) {
+                const int idx = w + l;
+                if (idx < U) dvals[pick<int32_t, U>(s0, idx)] = pick<A, U>(res, idx);
+            }
+            j += U;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c0[u] = c1[u];
+                s0[u] = s1[u];
+            }
         }
     }
     for (; j < len; ++j) {
         const int64_t e = base + static_cast<int64_t>(j) * C;
-        A acc = T::fma_(g, X[static_cast<int64_t>(col[e]) * K + v], A(0));
+        A xv[Q];
+        load_run<A, Q>(X + static_cast<int64_t>(col[e]) * K + l * Q, xv);
+        A acc = 0;
 #pragma unroll
-        for (int off = K / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, K);
-        if (v == 0) dvals[src[e]] = acc;
+        for (int t = 0; t < Q; ++t) acc = T::fma_(g[t], xv[t], acc);
+        acc = group_sum<GL>(acc);
+        if (l == 0) dvals[src[e]] = acc;
     }
 }
 
@@ -140,7 +223,7 @@ static void launch_sddmm_rows(smax_hip_ctx ctx, smax_hip_scs A, const void *Gm,
                               const void *X, void *dv) {
     using T = vt<VT>;
     const smax_hip_scs_info &in = A->info;
-    const int64_t ck = in.C * K;
+    const int64_t ck = in.C * sddmm_group(K);
     const int64_t cpb = ck <= 256 ? 256 / ck : 1;
     const int64_t bpc = ck <= 256 ? 1 : ceil_div(ck, 256);
     const int64_t blocks = cpb > 1 ? ceil_div(in.n_chunks, cpb) : in.n_chunks * bpc;
@@ -261,7 +344,11 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                      "smax_hip_scs_sddmm: bad arguments");
     const smax_hip_scs_info &in = A->info;
     if (in.n_elements == 0) return SMAX_HIP_SUCCESS;
-    if (in.C * k <= INT32_MAX && (k & (k - 1)) == 0 && k <= 64) {
+    // the row kernel's 16-byte loads need 16-byte aligned G and X rows
+    const int64_t run_bytes = sddmm_run(k) * (value_type == SMAX_HIP_F64 ? 8 : 4);
+    const bool aligned = run_bytes % 16 != 0 ||
+                         ((reinterpret_cast<uintptr_t>(G) | reinterpret_cast<uintptr_t>(X)) & 15) == 0;
+    if (aligned && (k & (k - 1)) == 0 && k <= 64 && in.C * sddmm_group(k) <= INT32_MAX) {
         bool done = false;
         SMAX_HIP_DISPATCH_VT(value_type, done = sddmm_rows<VT>(ctx, A, k, G, X, dvals));
         if (done) {
@@ -278,8 +365,7 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                            in.chunk_lengths, in.slot_len, in.slot_row, A->elem_row);
         SMAX_HIP_LAUNCHED(ctx, "k_elem_row");
     }
-    int g = 1;
-    while (g < k && g < 64) g <<= 1;
+    const int g = static_cast<int>(sddmm_group(k));
     SMAX_HIP_DISPATCH_VT(value_type,
                          return launch_sddmm<VT>(ctx, in, A->elem_row, k, g, G, X,
                                                  dvals));

@@ -51,6 +51,47 @@ def test_transpose_refresh_follows_new_values(ctx):
     T.destroy()
 
 
+def sddmm_tree(k):
+    """The SDDMM reduction tree for k: g lanes per element, each summing a
+    run of q consecutive v, then a butterfly over the g lanes."""
+    p = 1
+    while p < k:
+        p *= 2
+    g = min(64, max(1, p // 4))
+    return g, -(-k // g)
+
+
+@pytest.mark.parametrize("vt,k", [(capi.F32, 32), (capi.F64, 32), (capi.F32, 8),
+                                  (capi.BF16, 64), (capi.F64, 2)])
+def test_sddmm_row_and_element_kernels_agree(ctx, vt, k):
+    """With X 16-byte aligned a power-of-two k takes the row kernel (16-byte
+    loads); shifted by one entry it takes the per-element kernel.  Both
+    evaluate the same tree, so dvals agree bit for bit."""
+    A = matgen.lomax_band(6000, half=700)
+    val = {capi.F64: A.val, capi.F32: A.val.astype(np.float32),
+           capi.BF16: matgen.to_bf16(A.val)}[vt]
+    rp, col, v = upload_crs(ctx, A, val)
+    S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, val.dtype.itemsize, 64, 256)
+    vec = np.float64 if vt == capi.F64 else np.float32
+    G = matgen.uniform_f64(16, A.n_rows * k).astype(vec)
+    X = matgen.uniform_f64(17, A.n_cols * k).astype(vec)
+    dG = ctx.upload(G)
+    out = []
+    for shift in (0, 1):
+        # shift 1: X starts one entry (4 or 8 bytes) past a 16-byte boundary
+        dX = ctx.upload(np.concatenate([X[:1], X]) if shift else X)
+        assert dX.ptr % 16 == 0
+        dv = ctx.zeros(A.nnz, vec)
+        S.sddmm(vt, k, dG, dX.ptr + shift * X.itemsize, dv)
+        out.append(ctx.download(dv))
+        dv.free()
+        dX.free()
+    assert_bitwise(out[0], out[1], "row kernel vs element kernel")
+    for d in (dG, rp, col, v):
+        d.free()
+    S.destroy()
+
+
 @pytest.mark.parametrize("vt,k,C_", [(capi.F64, 1, 64), (capi.F64, 5, 64), (capi.F64, 32, 64),
                                      (capi.F32, 1, 64), (capi.F32, 32, 64), (capi.BF16, 32, 64),
                                      (capi.F32, 8, 7), (capi.F32, 64, 64), (capi.F64, 3, 32),
@@ -83,7 +124,9 @@ def test_sddmm(ctx, vt, k, C_):
         # one product, rounded once
         assert_bitwise(got, G[rows] * X[A.col], "k=1")
     u = np.finfo(vec).eps / 2
-    bound = (np.ceil(np.log2(k)) + 1) * u * scale * (1 + 4 * u) + 1e-300
+    g, q = sddmm_tree(k)
+    # q - 1 chained fmas per lane, log2(g) butterfly additions, one product rounding
+    bound = (q + np.log2(g)) * u * scale * (1 + 4 * u) + 1e-300
     assert (np.abs(got.astype(np.longdouble) - exact) <= bound).all()
     # the oracle's FP64 sums agree within their own rounding and ours
     assert (np.abs(got.astype(np.float64) - want) <=
