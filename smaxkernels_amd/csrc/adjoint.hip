@@ -89,26 +89,6 @@ __global__ __launch_bounds__(256) void scs_sddmm_kernel(
     if (s >= 0 && lane == 0) dvals[s] = acc;
 }
 
-// Q consecutive entries from p; with Q * sizeof(A) a multiple of 16 the host
-// has checked that p is 16-byte aligned, and they come as 16-byte loads.
-template <typename A, int Q>
-__device__ __forceinline__ void load_run(const A *__restrict__ p, A (&out)[Q]) {
-    if constexpr ((Q * sizeof(A)) % 16 == 0) {
-        constexpr int W = 16 / sizeof(A);
-        typedef A V __attribute__((ext_vector_type(W)));
-        const V *q = reinterpret_cast<const V *>(p);
-#pragma unroll
-        for (int h = 0; h < Q / W; ++h) {
-            const V w = q[h];
-#pragma unroll
-            for (int i = 0; i < W; ++i) out[h * W + i] = w[i];
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) out[i] = p[i];
-    }
-}
-
 // k = K a power of two <= 64: GL = sddmm_group(K) lanes per slot (row) in
 // SpMM order, long chunks first.  Lane l keeps G[r, l*Q .. l*Q+Q) in
 // registers for the whole row; each element costs one coalesced X row read
@@ -346,8 +326,7 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
     if (in.n_elements == 0) return SMAX_HIP_SUCCESS;
     // the row kernel's 16-byte loads need 16-byte aligned G and X rows
     const int64_t run_bytes = sddmm_run(k) * (value_type == SMAX_HIP_F64 ? 8 : 4);
-    const bool aligned = run_bytes % 16 != 0 ||
-                         ((reinterpret_cast<uintptr_t>(G) | reinterpret_cast<uintptr_t>(X)) & 15) == 0;
+    const bool aligned = run_bytes % 16 != 0 || aligned16(G, X);
     if (aligned && (k & (k - 1)) == 0 && k <= 64 && in.C * sddmm_group(k) <= INT32_MAX) {
         bool done = false;
         SMAX_HIP_DISPATCH_VT(value_type, done = sddmm_rows<VT>(ctx, A, k, G, X, dvals));

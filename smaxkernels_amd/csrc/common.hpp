@@ -105,6 +105,50 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks) {
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
+// Q consecutive entries at p.  When Q * sizeof(A) is a multiple of 16 the
+// caller guarantees p is 16-byte aligned, and they move as 16-byte accesses.
+template <typename A, int Q>
+__device__ __forceinline__ void load_run(const A *__restrict__ p, A (&out)[Q]) {
+    if constexpr ((Q * sizeof(A)) % 16 == 0) {
+        constexpr int W = 16 / sizeof(A);
+        typedef A V __attribute__((ext_vector_type(W)));
+        const V *q = reinterpret_cast<const V *>(p);
+#pragma unroll
+        for (int h = 0; h < Q / W; ++h) {
+            const V w = q[h];
+#pragma unroll
+            for (int i = 0; i < W; ++i) out[h * W + i] = w[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) out[i] = p[i];
+    }
+}
+
+template <typename A, int Q>
+__device__ __forceinline__ void store_run(A *__restrict__ p, const A (&in)[Q]) {
+    if constexpr ((Q * sizeof(A)) % 16 == 0) {
+        constexpr int W = 16 / sizeof(A);
+        typedef A V __attribute__((ext_vector_type(W)));
+        V *q = reinterpret_cast<V *>(p);
+#pragma unroll
+        for (int h = 0; h < Q / W; ++h) {
+            V w;
+#pragma unroll
+            for (int i = 0; i < W; ++i) w[i] = in[h * W + i];
+            q[h] = w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) p[i] = in[i];
+    }
+}
+
+// Whether every pointer is 16-byte aligned (the run helpers' precondition).
+inline bool aligned16(const void *a, const void *b) {
+    return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
+}
+
 // Streamed-once loads bypass cache allocation.
 template <typename T> __device__ __forceinline__ T ld_stream(const T *p) {
     return __builtin_nontemporal_load(p);

@@ -194,6 +194,115 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_kernel(
     else Y[v * ldy + r] = acc;
 }
 
+// Row-major SpMM with k = K a power of two and 16-byte aligned X/Y rows: a
+// lane owns Q = 16 / sizeof(vec) consecutive columns v (K / Q lanes per row),
+// reads its part of each X row with one 16-byte load and keeps Q fma chains;
+// the chain is software-pipelined U elements deep.  Every Y[r, v] is the
+// same sequential chain over the row as in crs_spmm_kernel.
+template <int VT, int K>
+__global__ __launch_bounds__(kCrsThreads) void crs_spmm_rowvec(
+    int64_t n_rows, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+    const typename vt<VT>::val_t *__restrict__ val,
+    const typename vt<VT>::vec_t *__restrict__ X, int64_t ldx,
+    typename vt<VT>::vec_t *__restrict__ Y, int64_t ldy) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    using V = typename T::val_t;
+    constexpr int Q = 16 / sizeof(A);
+    constexpr int GL = K / Q;
+    constexpr int U = 8;
+    static_assert(GL * Q == K, "K must be a multiple of the run length");
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kCrsThreads + threadIdx.x;
+    const int64_t r = t / GL;
+    const int l = static_cast<int>(t % GL);
+    if (r >= n_rows) return;
+    const int64_t s = row_ptr[r], e = row_ptr[r + 1];
+    A acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] = 0;
+    int64_t j = s;
+    if (e - s >= U) {
+        int32_t c0[U];
+        V w0[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c0[u] = col[j + u];
+            w0[u] = val[j + u];
+        }
+        for (;;) {
+            const bool more = j + 2 * U <= e;
+            int32_t c1[U];
+            V w1[U];
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    c1[u] = col[j + U + u];
+                    w1[u] = val[j + U + u];
+                }
+            }
+            A xv[U][Q];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                load_run<A, Q>(X + static_cast<int64_t>(c0[u]) * ldx + l * Q, xv[u]);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc[q] = T::fma_(T::widen(w0[u]), xv[u][q], acc[q]);
+            j += U;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c0[u] = c1[u];
+                w0[u] = w1[u];
+            }
+        }
+    }
+    for (; j < e; ++j) {
+        A xv[Q];
+        load_run<A, Q>(X + static_cast<int64_t>(col[j]) * ldx + l * Q, xv);
+        const A w = T::widen(val[j]);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] = T::fma_(w, xv[q], acc[q]);
+    }
+    store_run<A, Q>(Y + r * ldy + l * Q, acc);
+}
+
+// Launches crs_spmm_rowvec when k is a power of two in [Q, 64] and X, Y and
+// their row strides are 16-byte aligned; false otherwise.
+template <int VT>
+static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
+                                   const int32_t *row_ptr, const int32_t *col,
+                                   const void *val, const void *X, int64_t ldx, void *Y,
+                                   int64_t ldy) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    constexpr int64_t Q = 16 / sizeof(A);
+    if (k < Q || k > 64 || (k & (k - 1)) != 0 || !aligned16(X, Y) ||
+        (ldx * sizeof(A)) % 16 != 0 || (ldy * sizeof(A)) % 16 != 0)
+        return false;
+    const dim3 g = grid_for(n_rows * (k / Q), kCrsThreads);
+    auto v = static_cast<const typename T::val_t *>(val);
+    auto xv = static_cast<const A *>(X);
+    auto yv = static_cast<A *>(Y);
+#define SMAX_CRS_ROWVEC(KK)                                                                  \
+    case KK:                                                                                  \
+        if constexpr (KK >= Q)                                                                \
+            hipLaunchKernelGGL((crs_spmm_rowvec<VT, KK>), g, dim3(kCrsThreads), 0, ctx->stream, \
+                               n_rows, row_ptr, col, v, xv, ldx, yv, ldy);                    \
+        return true;
+    switch (k) {
+        SMAX_CRS_ROWVEC(2)
+        SMAX_CRS_ROWVEC(4)
+        SMAX_CRS_ROWVEC(8)
+        SMAX_CRS_ROWVEC(16)
+        SMAX_CRS_ROWVEC(32)
+        SMAX_CRS_ROWVEC(64)
+    default:
+        return false;
+    }
+#undef SMAX_CRS_ROWVEC
+}
+
 }  // namespace smaxhip
 
 using namespace smaxhip;
@@ -340,6 +449,11 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
         auto v = static_cast<const typename T::val_t *>(val);
         auto xv = static_cast<const typename T::vec_t *>(X);
         auto yv = static_cast<typename T::vec_t *>(Y);
+        if (row_major && launch_crs_spmm_rowvec<VT>(ctx, n_rows, k, row_ptr, col, val, X, ldx,
+                                                    Y, ldy)) {
+            SMAX_HIP_LAUNCHED(ctx, "crs_spmm_rowvec");
+            return SMAX_HIP_SUCCESS;
+        }
         if (row_major)
             hipLaunchKernelGGL((crs_spmm_kernel<VT, true>),
                                grid_for(threads, kCrsThreads), dim3(kCrsThreads),

@@ -272,6 +272,113 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmm_kernel(
     Y[static_cast<int64_t>(r) * k + v] = acc;
 }
 
+// As scs_spmm_kernel for k = K a power of two with 16-byte aligned X and Y:
+// a lane owns Q = 16 / sizeof(vec) consecutive columns v (K / Q lanes per
+// slot), reads its part of each X row with one 16-byte load and keeps Q fma
+// chains, each the same sequential chain as in scs_spmm_kernel.
+template <int VT, bool LEN, int K>
+__global__ __launch_bounds__(kScsThreads) void scs_spmm_vec(
+    int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
+    const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ chunk_len,
+    const int32_t *__restrict__ slot_len, const int32_t *__restrict__ order,
+    const int32_t *__restrict__ col, const typename vt<VT>::val_t *__restrict__ val,
+    const int32_t *__restrict__ slot_row, const typename vt<VT>::vec_t *__restrict__ X,
+    typename vt<VT>::vec_t *__restrict__ Y) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    using V = typename T::val_t;
+    constexpr int Q = 16 / sizeof(A);
+    constexpr int GL = K / Q;
+    constexpr int U = 8;
+    static_assert(GL * Q == K, "K must be a multiple of the run length");
+    const int64_t ck = C * GL;
+    int64_t pos, li;
+    if (cpb > 1) {
+        if (threadIdx.x >= cpb * ck) return;
+        pos = static_cast<int64_t>(blockIdx.x) * cpb + threadIdx.x / ck;
+        li = threadIdx.x % ck;
+    } else {
+        pos = blockIdx.x / bpc;
+        li = (blockIdx.x % bpc) * kScsThreads + threadIdx.x;
+    }
+    if (pos >= n_chunks || li >= ck) return;
+    const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
+    const int64_t i = li / GL;
+    const int l = static_cast<int>(li % GL);
+    const int64_t slot = chunk * C + i;
+    const int32_t r = slot_row[slot];
+    if (r >= n_rows) return;
+    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
+    const int len = LEN ? slot_len[slot] : chunk_len[chunk];
+    A acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] = 0;
+    int j = 0;
+    if (len >= U) {
+        int32_t c0[U];
+        V w0[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c0[u] = col[base + u * C];
+            w0[u] = val[base + u * C];
+        }
+        for (;;) {
+            const bool more = j + 2 * U <= len;
+            int32_t c1[U];
+            V w1[U];
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    c1[u] = col[base + static_cast<int64_t>(j + U + u) * C];
+                    w1[u] = val[base + static_cast<int64_t>(j + U + u) * C];
+                }
+            }
+            A xv[U][Q];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                load_run<A, Q>(X + static_cast<int64_t>(c0[u]) * K + l * Q, xv[u]);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc[q] = T::fma_(T::widen(w0[u]), xv[u][q], acc[q]);
+            j += U;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c0[u] = c1[u];
+                w0[u] = w1[u];
+            }
+        }
+    }
+    for (; j < len; ++j) {
+        const int64_t o = base + static_cast<int64_t>(j) * C;
+        A xv[Q];
+        load_run<A, Q>(X + static_cast<int64_t>(col[o]) * K + l * Q, xv);
+        const A w = T::widen(val[o]);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] = T::fma_(w, xv[q], acc[q]);
+    }
+    store_run<A, Q>(Y + static_cast<int64_t>(r) * K + l * Q, acc);
+}
+
+template <int VT, int K>
+static void launch_scs_spmm_vec(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_t n_rows,
+                                int64_t cpb, int64_t bpc, int64_t blocks,
+                                const int32_t *chunk_ptr, const int32_t *chunk_len,
+                                const int32_t *slot_len, const int32_t *order,
+                                const int32_t *col, const typename vt<VT>::val_t *val,
+                                const int32_t *slot_row, const typename vt<VT>::vec_t *X,
+                                typename vt<VT>::vec_t *Y) {
+    if (slot_len)
+        hipLaunchKernelGGL((scs_spmm_vec<VT, true, K>), dim3(static_cast<unsigned>(blocks)),
+                           dim3(kScsThreads), 0, ctx->stream, C, n_chunks, n_rows, cpb, bpc,
+                           chunk_ptr, chunk_len, slot_len, order, col, val, slot_row, X, Y);
+    else
+        hipLaunchKernelGGL((scs_spmm_vec<VT, false, K>), dim3(static_cast<unsigned>(blocks)),
+                           dim3(kScsThreads), 0, ctx->stream, C, n_chunks, n_rows, cpb, bpc,
+                           chunk_ptr, chunk_len, slot_len, order, col, val, slot_row, X, Y);
+}
+
 template <int VT>
 static int launch_scs_spmm(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_t n_rows,
                            int64_t k, const int32_t *chunk_ptr, const int32_t *chunk_len,
@@ -279,7 +386,11 @@ static int launch_scs_spmm(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_
                            const int32_t *col, const void *val, const int32_t *slot_row,
                            const void *X, void *Y) {
     using T = vt<VT>;
-    const int64_t ck = C * k;
+    using A = typename T::vec_t;
+    constexpr int64_t Q = 16 / sizeof(A);
+    // k a power of two in [Q, 64] with 16-byte aligned operands: Q columns per lane
+    const bool vec = k >= Q && k <= 64 && (k & (k - 1)) == 0 && aligned16(X, Y);
+    const int64_t ck = C * (vec ? k / Q : k);
     const int64_t cpb = ck <= kScsThreads ? kScsThreads / ck : 1;
     const int64_t bpc = ck <= kScsThreads ? 1 : ceil_div(ck, kScsThreads);
     const int64_t blocks = cpb > 1 ? ceil_div(n_chunks, cpb) : n_chunks * bpc;
@@ -287,6 +398,25 @@ static int launch_scs_spmm(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_
     auto v = static_cast<const typename T::val_t *>(val);
     auto xv = static_cast<const typename T::vec_t *>(X);
     auto yv = static_cast<typename T::vec_t *>(Y);
+    if (vec) {
+        switch (k) {
+#define SMAX_SCS_SPMM_VEC(KK)                                                                \
+    case KK:                                                                                  \
+        if constexpr (KK >= Q)                                                                \
+            launch_scs_spmm_vec<VT, KK>(ctx, C, n_chunks, n_rows, cpb, bpc, blocks, chunk_ptr, \
+                                        chunk_len, slot_len, order, col, v, slot_row, xv, yv); \
+        break;
+            SMAX_SCS_SPMM_VEC(2)
+            SMAX_SCS_SPMM_VEC(4)
+            SMAX_SCS_SPMM_VEC(8)
+            SMAX_SCS_SPMM_VEC(16)
+            SMAX_SCS_SPMM_VEC(32)
+            SMAX_SCS_SPMM_VEC(64)
+#undef SMAX_SCS_SPMM_VEC
+        }
+        SMAX_HIP_LAUNCHED(ctx, "scs_spmm_vec");
+        return SMAX_HIP_SUCCESS;
+    }
     if (slot_len)
         hipLaunchKernelGGL((scs_spmm_kernel<VT, true>), dim3(static_cast<unsigned>(blocks)),
                            dim3(kScsThreads), 0, ctx->stream, C, n_chunks, n_rows, k, cpb,

@@ -119,6 +119,26 @@ def test_crs_spmm_bit_exact(ctx, vt, k, row_major):
                    oracle.crs_spmm(A.row_ptr, A.col, val, k, row_major, X))
 
 
+@pytest.mark.parametrize("vt", VTS)
+def test_crs_spmm_row_major_unaligned_operands(ctx, vt):
+    """k = 32 row-major with X and Y one entry past a 16-byte boundary takes
+    the one-column-per-lane kernel instead of the 16-byte one; both run the
+    same chains, so both equal the reference."""
+    k = 32
+    A = matgen.lomax_band(8000, half=900)
+    val, X = operands(vt, A, k)
+    rp, col, v = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(val)
+    dX = ctx.upload(np.concatenate([X[:1], X]))
+    dY = ctx.zeros(A.n_rows * k + 1, X.dtype)
+    it = X.itemsize
+    ctx.crs_spmm(vt, A.n_rows, A.n_cols, A.nnz, rp, col, v, k, True, dX.ptr + it, k,
+                 dY.ptr + it, k)
+    assert_bitwise(ctx.download(dY)[1:],
+                   oracle.crs_spmm(A.row_ptr, A.col, val, k, True, X))
+    for d in (rp, col, v, dX, dY):
+        d.free()
+
+
 def test_kat_spmm_on_gpu(ctx):
     kats = reference_kats()
     for name, row_major in (("spmm_test.hpp", False), ("spmm_rowmaj_test.hpp", True)):

@@ -271,7 +271,7 @@ def test_scs_spmv_config1_poisson(ctx):
 
 
 @pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
-@pytest.mark.parametrize("k", [1, 3, 32])
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 32, 64])
 def test_scs_spmm_bit_exact(ctx, vt, k):
     A = matgen.lomax_band(20000, half=2500)
     val, _ = values_for(vt, A)
