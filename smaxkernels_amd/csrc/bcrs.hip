@@ -145,13 +145,15 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_kernel(
     Y[(r * h_pad + ix) * k + v] = acc;
 }
 
-// Register-blocked SpMM for compile-time block shapes: thread (r, v) keeps
-// the BH outputs of block row r in column v, so each X row of a block is
-// loaded once for all BH rows (instead of once per row), and the BH chains
-// give independent fmas.  Per output the fma order is still the reference's
-// (block j, then jx); the next block's values and X entries are loaded while
-// this block's fmas run.
-template <int VT, int BH, int BW>
+// Register-blocked SpMM for compile-time block shapes: thread (r, l) keeps
+// the BH outputs of block row r in columns v = l*Q .. l*Q+Q-1, so each X row
+// of a block is loaded once for all BH rows (instead of once per row), and
+// the BH*Q chains give independent fmas.  Q > 1 (16-byte runs: k a multiple
+// of Q, X and Y 16-byte aligned) reads each lane's part of an X row with one
+// 16-byte load.  Per output the fma order is still the reference's (block j,
+// then jx); the next block's values and X entries are loaded while this
+// block's fmas run.
+template <int VT, int BH, int BW, int Q>
 __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     int64_t n_block_rows, int h_pad, int w_pad, int col_major, int64_t k,
     const int32_t *__restrict__ col, const int32_t *__restrict__ row_ptr,
@@ -161,37 +163,40 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     using T = vt<VT>;
     using A = typename T::vec_t;
     using V = typename T::val_t;
+    const int64_t gl = k / Q;  // lanes per block row
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
-    const int64_t r = t / k, v = t - r * k;
+    const int64_t r = t / gl, v = (t - r * gl) * Q;
     if (r >= n_block_rows) return;
     const int64_t bs = static_cast<int64_t>(h_pad) * w_pad;
     const int64_t vs_i = col_major ? 1 : w_pad, vs_j = col_major ? h_pad : 1;
-    A acc[BH];
+    A acc[BH][Q];
 #pragma unroll
-    for (int ix = 0; ix < BH; ++ix) acc[ix] = 0;
+    for (int ix = 0; ix < BH; ++ix)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[ix][q] = 0;
     int64_t j = row_ptr[r];
     const int64_t je = row_ptr[r + 1];
     if (j < je) {
         V w0[BH][BW];
-        A x0[BW];
+        A x0[BW][Q];
         {
             const V *blk = val + j * bs;
             const A *xr = X + static_cast<int64_t>(col[j]) * w_pad * k + v;
 #pragma unroll
             for (int jx = 0; jx < BW; ++jx) {
-                x0[jx] = xr[jx * k];
+                load_run<A, Q>(xr + jx * k, x0[jx]);
 #pragma unroll
                 for (int ix = 0; ix < BH; ++ix) w0[ix][jx] = blk[ix * vs_i + jx * vs_j];
             }
         }
         for (; j + 1 < je; ++j) {
             V w1[BH][BW];
-            A x1[BW];
+            A x1[BW][Q];
             const V *blk = val + (j + 1) * bs;
             const A *xr = X + static_cast<int64_t>(col[j + 1]) * w_pad * k + v;
 #pragma unroll
             for (int jx = 0; jx < BW; ++jx) {
-                x1[jx] = xr[jx * k];
+                load_run<A, Q>(xr + jx * k, x1[jx]);
 #pragma unroll
                 for (int ix = 0; ix < BH; ++ix) w1[ix][jx] = blk[ix * vs_i + jx * vs_j];
             }
@@ -199,10 +204,13 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
             for (int ix = 0; ix < BH; ++ix)
 #pragma unroll
                 for (int jx = 0; jx < BW; ++jx)
-                    acc[ix] = T::fma_(T::widen(w0[ix][jx]), x0[jx], acc[ix]);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+                        acc[ix][q] = T::fma_(T::widen(w0[ix][jx]), x0[jx][q], acc[ix][q]);
 #pragma unroll
             for (int jx = 0; jx < BW; ++jx) {
-                x0[jx] = x1[jx];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) x0[jx][q] = x1[jx][q];
 #pragma unroll
                 for (int ix = 0; ix < BH; ++ix) w0[ix][jx] = w1[ix][jx];
             }
@@ -211,10 +219,12 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
         for (int ix = 0; ix < BH; ++ix)
 #pragma unroll
             for (int jx = 0; jx < BW; ++jx)
-                acc[ix] = T::fma_(T::widen(w0[ix][jx]), x0[jx], acc[ix]);
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    acc[ix][q] = T::fma_(T::widen(w0[ix][jx]), x0[jx][q], acc[ix][q]);
     }
 #pragma unroll
-    for (int ix = 0; ix < BH; ++ix) Y[(r * h_pad + ix) * k + v] = acc[ix];
+    for (int ix = 0; ix < BH; ++ix) store_run<A, Q>(Y + (r * h_pad + ix) * k + v, acc[ix]);
 }
 
 template <int VT, int BH, int BW>
@@ -223,12 +233,23 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
                                  const int32_t *row_ptr, const void *val, const void *X,
                                  void *Y) {
     using T = vt<VT>;
-    hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW>), grid_for(n_block_rows * k, kBThreads),
-                       dim3(kBThreads), 0, ctx->stream, n_block_rows,
-                       static_cast<int>(h_pad), static_cast<int>(w_pad), cm ? 1 : 0, k,
-                       col, row_ptr, static_cast<const typename T::val_t *>(val),
-                       static_cast<const typename T::vec_t *>(X),
-                       static_cast<typename T::vec_t *>(Y));
+    using A = typename T::vec_t;
+    constexpr int Q = 16 / sizeof(A);
+    auto v = static_cast<const typename T::val_t *>(val);
+    auto xv = static_cast<const A *>(X);
+    auto yv = static_cast<A *>(Y);
+    // 16-byte runs need at least 4 lanes per block row: with fewer (k = 8 in
+    // FP32) the per-lane chains run longer than the saved instructions pay for
+    if (k % Q == 0 && k >= 4 * Q && aligned16(X, Y))
+        hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW, Q>),
+                           grid_for(n_block_rows * (k / Q), kBThreads), dim3(kBThreads), 0,
+                           ctx->stream, n_block_rows, static_cast<int>(h_pad),
+                           static_cast<int>(w_pad), cm ? 1 : 0, k, col, row_ptr, v, xv, yv);
+    else
+        hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW, 1>), grid_for(n_block_rows * k, kBThreads),
+                           dim3(kBThreads), 0, ctx->stream, n_block_rows,
+                           static_cast<int>(h_pad), static_cast<int>(w_pad), cm ? 1 : 0, k,
+                           col, row_ptr, v, xv, yv);
 }
 
 template <int VT, int BW>

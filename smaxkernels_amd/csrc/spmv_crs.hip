@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_rowvec(
     store_run<A, Q>(Y + r * ldy + l * Q, acc);
 }
 
-// Launches crs_spmm_rowvec when k is a power of two in [Q, 64] and X, Y and
+// Launches crs_spmm_rowvec when k is a power of two in [4Q, 64] and X, Y and
 // their row strides are 16-byte aligned; false otherwise.
 template <int VT>
 static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
@@ -277,7 +277,8 @@ static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
     using T = vt<VT>;
     using A = typename T::vec_t;
     constexpr int64_t Q = 16 / sizeof(A);
-    if (k < Q || k > 64 || (k & (k - 1)) != 0 || !aligned16(X, Y) ||
+    // at least 4 lanes per row (as for the BCRS SpMM, where 2 lost time)
+    if (k < 4 * Q || k > 64 || (k & (k - 1)) != 0 || !aligned16(X, Y) ||
         (ldx * sizeof(A)) % 16 != 0 || (ldy * sizeof(A)) % 16 != 0)
         return false;
     const dim3 g = grid_for(n_rows * (k / Q), kCrsThreads);

@@ -388,8 +388,9 @@ static int launch_scs_spmm(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_
     using T = vt<VT>;
     using A = typename T::vec_t;
     constexpr int64_t Q = 16 / sizeof(A);
-    // k a power of two in [Q, 64] with 16-byte aligned operands: Q columns per lane
-    const bool vec = k >= Q && k <= 64 && (k & (k - 1)) == 0 && aligned16(X, Y);
+    // k a power of two in [4Q, 64] with 16-byte aligned operands: Q columns
+    // per lane, at least 4 lanes per slot
+    const bool vec = k >= 4 * Q && k <= 64 && (k & (k - 1)) == 0 && aligned16(X, Y);
     const int64_t ck = C * (vec ? k / Q : k);
     const int64_t cpb = ck <= kScsThreads ? kScsThreads / ck : 1;
     const int64_t bpc = ck <= kScsThreads ? 1 : ceil_div(ck, kScsThreads);

@@ -96,15 +96,19 @@ def test_block7_conversion_spmv_spmm(ctx, vt, bh, bw, hp, wp, cm):
                           want["row_ptr"], want["val"], cm, x)
     rows = (np.arange(i.n_block_rows * hp) % hp) < bh
     assert_bitwise(ctx.download(dy)[rows], yw[rows], "bcrs spmv")
-    for k in (1, 8, 64):
+    # shift 1: X and Y one entry past a 16-byte boundary (one column per lane
+    # instead of 16-byte runs; the same chains)
+    for k, shift in ((1, 0), (6, 0), (8, 0), (8, 1), (64, 0)):
         X = matgen.uniform_f64(4, i.n_block_cols * wp * k).astype(xt)
-        dX, dY = ctx.upload(X), ctx.zeros(i.n_block_rows * hp * k, xt)
+        n_y = i.n_block_rows * hp * k
+        dX = ctx.upload(np.concatenate([X[:shift], X]))
+        dY = ctx.zeros(n_y + shift, xt)
         ctx.bcrs_spmm(vt, i.n_block_rows, bh, bw, hp, wp, i.col, i.row_ptr, i.val,
-                      cm, k, dX, dY)
+                      cm, k, dX.ptr + shift * X.itemsize, dY.ptr + shift * X.itemsize)
         Yw = oracle.bcrs_spmm(i.n_block_rows, bh, bw, hp, wp, want["col"],
                               want["row_ptr"], want["val"], cm, k, X)
         rk = np.repeat(rows, k)
-        assert_bitwise(ctx.download(dY)[rk], Yw[rk], f"bcrs spmm k={k}")
+        assert_bitwise(ctx.download(dY)[shift:][rk], Yw[rk], f"bcrs spmm k={k} shift={shift}")
         dX.free()
         dY.free()
     b.destroy()
