@@ -118,27 +118,34 @@ __device__ __forceinline__ float fma_w(float a, float b, float c) {
 }
 
 // (A x)[s]: sequential fma chain over the slot's SELL elements, then (DIAG)
-// the FP32 diagonal term.
-template <typename V, typename W, bool DIAG>
+// the FP32 diagonal term.  REL: column indices come as 16-bit offsets from
+// the slot (rel[e] = col_slot[e] - s; padding elements hold 0 and read the
+// slot's own x entry, times a zero value).
+template <typename V, typename W, bool DIAG, bool REL>
 __device__ __forceinline__ W apply_slot(int64_t s, int64_t C,
                                         const int32_t *__restrict__ chunk_ptr,
                                         const int32_t *__restrict__ chunk_len,
                                         const int32_t *__restrict__ col,
+                                        const int16_t *__restrict__ rel,
                                         const V *__restrict__ val,
                                         const float *__restrict__ diag,
                                         const W *__restrict__ x) {
     const int64_t c = s / C;
     const int64_t base = static_cast<int64_t>(chunk_ptr[c]) + (s - c * C);
     const int len = chunk_len[c];
+    auto col_at = [&](int64_t o) -> int64_t {
+        if constexpr (REL) return s + ld_stream(rel + o);
+        else return ld_stream(col + o);
+    };
     W acc = 0;
     constexpr int U = 8;
     int j = 0;
     for (; j + U <= len; j += U) {
-        int32_t cc[U];
+        int64_t cc[U];
         V vv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            cc[u] = ld_stream(col + base + (j + u) * C);
+            cc[u] = col_at(base + (j + u) * C);
             vv[u] = ld_stream(val + base + (j + u) * C);
         }
         W xv[U];
@@ -150,25 +157,25 @@ __device__ __forceinline__ W apply_slot(int64_t s, int64_t C,
     }
     for (; j < len; ++j)
         acc = fma_w(static_cast<W>(widen_t<V>::f(val[base + j * C])),
-                    x[col[base + j * C]], acc);
+                    x[col_at(base + j * C)], acc);
     if (DIAG) acc = fma_w(static_cast<W>(diag[s]), x[s], acc);
     return acc;
 }
 
 // q = A p, partial[block] = sum p.q
-template <typename V, typename W, bool DIAG>
+template <typename V, typename W, bool DIAG, bool REL>
 __global__ __launch_bounds__(kCgThreads) void k_spmv_dot(
     int64_t n, int64_t C, const int32_t *__restrict__ chunk_ptr,
     const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ col,
-    const V *__restrict__ val, const float *__restrict__ diag,
-    const W *__restrict__ p, W *__restrict__ q, W *__restrict__ partial,
-    const int64_t *__restrict__ ctl) {
+    const int16_t *__restrict__ rel, const V *__restrict__ val,
+    const float *__restrict__ diag, const W *__restrict__ p, W *__restrict__ q,
+    W *__restrict__ partial, const int64_t *__restrict__ ctl) {
     if (ctl[CTL_DONE]) return;
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
     W prod = 0;
     if (s < n) {
-        const W v = apply_slot<V, W, DIAG>(s, C, chunk_ptr, chunk_len, col, val,
-                                           diag, p);
+        const W v = apply_slot<V, W, DIAG, REL>(s, C, chunk_ptr, chunk_len, col, rel, val,
+                                                diag, p);
         q[s] = v;
         prod = p[s] * v;
     }
@@ -177,17 +184,18 @@ __global__ __launch_bounds__(kCgThreads) void k_spmv_dot(
 }
 
 // r = b - A x, partial = sum r.r  (FP64 outer residual)
+template <bool REL>
 __global__ __launch_bounds__(kCgThreads) void k_residual(
     int64_t n, int64_t C, const int32_t *__restrict__ chunk_ptr,
     const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ col,
-    const double *__restrict__ val, const double *__restrict__ b,
-    const double *__restrict__ x, double *__restrict__ r,
+    const int16_t *__restrict__ rel, const double *__restrict__ val,
+    const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r,
     double *__restrict__ partial) {
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
     double sq = 0;
     if (s < n) {
-        const double ax = apply_slot<double, double, false>(
-            s, C, chunk_ptr, chunk_len, col, val, nullptr, x);
+        const double ax = apply_slot<double, double, false, REL>(
+            s, C, chunk_ptr, chunk_len, col, rel, val, nullptr, x);
         const double v = b[s] - ax;
         r[s] = v;
         sq = v * v;
@@ -387,6 +395,40 @@ __global__ void k_col_slot(int64_t m, const int32_t *__restrict__ col,
     if (e < m) out[e] = perm[col[e]];
 }
 
+// rel[e] = col_slot[e] - s for the elements of slot s (0 for padding);
+// flags[block] = 1 when one of the block's offsets does not fit int16.
+__global__ __launch_bounds__(256) void k_col_rel(int64_t npad, int64_t C,
+                                                 const int32_t *__restrict__ chunk_ptr,
+                                                 const int32_t *__restrict__ chunk_len,
+                                                 const int32_t *__restrict__ col_slot,
+                                                 const int32_t *__restrict__ src,
+                                                 int16_t *__restrict__ rel,
+                                                 int32_t *__restrict__ flags) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int bad = 0;
+    if (s < npad) {
+        const int64_t c = s / C;
+        const int64_t base = static_cast<int64_t>(chunk_ptr[c]) + (s - c * C);
+        for (int j = 0; j < chunk_len[c]; ++j) {
+            const int64_t o = base + static_cast<int64_t>(j) * C;
+            const int64_t off = src[o] >= 0 ? col_slot[o] - s : 0;
+            bad |= off < -32768 || off > 32767;
+            rel[o] = static_cast<int16_t>(off);
+        }
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) flags[blockIdx.x] = bad;
+}
+
+// out[0] = OR of flags[0..m) (one workgroup).
+__global__ __launch_bounds__(1024) void k_or_flags(const int32_t *__restrict__ flags, int64_t m,
+                                                   int64_t *__restrict__ out) {
+    int any = 0;
+    for (int64_t i = threadIdx.x; i < m; i += 1024) any |= flags[i];
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) out[0] = any;
+}
+
 __global__ void k_round_f32(int64_t m, const double *__restrict__ v,
                             float *__restrict__ out) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -436,7 +478,23 @@ __global__ void k_split_bf16(int64_t npad, int64_t C,
 struct Mat {
     int64_t n, C;
     const int32_t *chunk_ptr, *chunk_len, *col;
+    const int16_t *rel;  // 16-bit column offsets from the slot, or nullptr
 };
+
+// q = A p with partial dots, through the matrix's column representation.
+template <typename V, typename W, bool DIAG>
+static void launch_spmv_dot(smax_hip_ctx ctx, dim3 sg, const Mat &M, const V *val,
+                            const float *diag, const W *p, W *q, W *partial,
+                            const int64_t *ctl) {
+    if (M.rel)
+        hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG, true>), sg, dim3(kCgThreads), 0,
+                           ctx->stream, M.n, M.C, M.chunk_ptr, M.chunk_len, M.col, M.rel,
+                           val, diag, p, q, partial, ctl);
+    else
+        hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG, false>), sg, dim3(kCgThreads), 0,
+                           ctx->stream, M.n, M.C, M.chunk_ptr, M.chunk_len, M.col, M.rel,
+                           val, diag, p, q, partial, ctl);
+}
 
 struct Work {
     smax_hip_ctx ctx;
@@ -493,9 +551,7 @@ static int cg_loop(smax_hip_ctx ctx, const Mat &A, const V *val, const float *di
         const int64_t nb = std::min<int64_t>(batch, max_it - done_it);
         for (int64_t b = 0, pp = par; b < nb; ++b, pp = 1 - pp) {
             const int cur = static_cast<int>(pp), nxt = 1 - cur;
-            hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG>), sg, dim3(kCgThreads), 0,
-                               ctx->stream, n, A.C, A.chunk_ptr, A.chunk_len, A.col,
-                               val, diag, p, q, partial, ctl);
+            launch_spmv_dot<V, W, DIAG>(ctx, sg, A, val, diag, p, q, partial, ctl);
             hipLaunchKernelGGL((k_reduce_pq<W>), dim3(1), dim3(kReduceThreads), 0,
                                ctx->stream, partial, static_cast<int64_t>(sg.x),
                                scal + 2, ctl);
@@ -530,8 +586,12 @@ static int residual(smax_hip_ctx ctx, const Mat &A, const double *val,
                     const double *b, const double *x, double *r, double *partial,
                     double *scal, int idx, double *rr) {
     const dim3 sg = grid_for(A.n, kCgThreads);
-    hipLaunchKernelGGL(k_residual, sg, dim3(kCgThreads), 0, ctx->stream, A.n, A.C,
-                       A.chunk_ptr, A.chunk_len, A.col, val, b, x, r, partial);
+    if (A.rel)
+        hipLaunchKernelGGL(k_residual<true>, sg, dim3(kCgThreads), 0, ctx->stream, A.n, A.C,
+                           A.chunk_ptr, A.chunk_len, A.col, A.rel, val, b, x, r, partial);
+    else
+        hipLaunchKernelGGL(k_residual<false>, sg, dim3(kCgThreads), 0, ctx->stream, A.n, A.C,
+                           A.chunk_ptr, A.chunk_len, A.col, A.rel, val, b, x, r, partial);
     hipLaunchKernelGGL((k_reduce<double>), dim3(1), dim3(kReduceThreads), 0, ctx->stream,
                        partial, static_cast<int64_t>(sg.x), scal + idx, nullptr, -1.0);
     SMAX_HIP_LAUNCHED(ctx, "k_residual");
@@ -565,6 +625,38 @@ static int prepare(smax_hip_ctx ctx, smax_hip_scs A, int lo_type, bool want_diag
         SMAX_HIP_LAUNCHED(ctx, "k_col_slot");
     }
     const int32_t *cs = A->col_slot ? A->col_slot : in.col;
+    if (A->col_rel_state == 0) {
+        // 16-bit slot-relative columns halve the index stream of every SpMV
+        // when the matrix's bandwidth in slot space allows (stencils do)
+        const int64_t nb = grid_for(in.n_rows_padded, 256).x;
+        int32_t *flags = nullptr;
+        int64_t *word = nullptr;
+        rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->col_rel), 2 * m, "cg col_rel");
+        if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&flags), 4 * nb, "cg col_rel");
+        if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&word), 8, "cg col_rel");
+        int64_t any = 1;
+        if (!rc) {
+            hipLaunchKernelGGL(k_col_rel, dim3(static_cast<unsigned>(nb)), dim3(256), 0,
+                               ctx->stream, in.n_rows_padded, in.C, in.chunk_ptr,
+                               in.chunk_lengths, cs, in.src_index, A->col_rel, flags);
+            hipLaunchKernelGGL(k_or_flags, dim3(1), dim3(1024), 0, ctx->stream, flags, nb,
+                               word);
+            rc = check_launch(ctx, "k_col_rel");
+            if (!rc) rc = read_words(ctx, word, &any, 1);
+        }
+        dev_free(flags);
+        dev_free(word);
+        if (rc) {
+            dev_free(A->col_rel);
+            A->col_rel = nullptr;
+            return rc;
+        }
+        if (any) {
+            dev_free(A->col_rel);
+            A->col_rel = nullptr;
+        }
+        A->col_rel_state = any ? -1 : 1;
+    }
     const double *v64 = static_cast<const double *>(in.val);
     if (lo_type == SMAX_HIP_F32 && !A->val_f32) {
         rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->val_f32), 4 * m, "cg f32");
@@ -651,9 +743,8 @@ static int reliable_cg(smax_hip_ctx ctx, const Mat &M, const double *val64,
             const int64_t nb = std::min<int64_t>(batch, cycle_end - done_it);
             for (int64_t b = 0, pp = done_it & 1; b < nb; ++b, pp = 1 - pp) {
                 const int cur = static_cast<int>(pp), nxt = 1 - cur;
-                hipLaunchKernelGGL((k_spmv_dot<V, float, DIAG>), sg, dim3(kCgThreads),
-                                   0, ctx->stream, n, M.C, M.chunk_ptr, M.chunk_len,
-                                   M.col, val, diag, p, q, partial32, ctl);
+                launch_spmv_dot<V, float, DIAG>(ctx, sg, M, val, diag, p, q, partial32,
+                                                ctl);
                 hipLaunchKernelGGL((k_reduce_pq<float>), dim3(1), dim3(kReduceThreads), 0,
                                    ctx->stream, partial32, static_cast<int64_t>(sg.x),
                                    scal32 + 2, ctl);
@@ -743,7 +834,7 @@ static int check_solver_args(smax_hip_ctx ctx, const char *fn, smax_hip_scs A,
 static Mat slot_matrix(smax_hip_scs A) {
     const smax_hip_scs_info &in = A->info;
     return Mat{in.n_rows_padded, in.C, in.chunk_ptr, in.chunk_lengths,
-               A->col_slot ? A->col_slot : in.col};
+               A->col_slot ? A->col_slot : in.col, A->col_rel};
 }
 
 }  // namespace smaxhip

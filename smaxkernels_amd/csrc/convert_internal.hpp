@@ -25,6 +25,11 @@ struct smax_hip_scs_s {
     uint16_t *val_bf16 = nullptr; // values rounded to bfloat16
     float *diag_f32 = nullptr;    // diagonal per slot, binary32
     int bf16_split = -1;          // 1: val_bf16 holds 0 on the diagonal
+    // Built on first use by the CG solvers when every stored element's slot-
+    // space column lies within +-32767 of its slot: col_slot - slot as int16
+    // (0 for padding).  col_rel_state: 0 not tried, 1 built, -1 does not fit.
+    int16_t *col_rel = nullptr;
+    int col_rel_state = 0;
     smax_hip_scs_sched sched;     // built with the conversion
     // Built on first use by the SDDMM: row of every stored element (-1 for
     // padding), so an element's row needs no search.

@@ -708,6 +708,7 @@ int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs) {
     }
     scs_free_arrays(scs->info);
     dev_free(scs->col_slot);
+    dev_free(scs->col_rel);
     dev_free(scs->val_f32);
     dev_free(scs->val_bf16);
     dev_free(scs->diag_f32);

@@ -42,6 +42,23 @@ def test_fp64_cg_reaches_1e10(ctx, C_, sigma):
     assert 10 < rep.iterations < 500
 
 
+@pytest.mark.parametrize("sigma", [1, 1 << 30])
+def test_cg_with_and_without_16bit_columns(ctx, sigma):
+    """40^3 stencil: at sigma = 1 every slot-space column lies within 1641 of
+    its slot, so the solvers read 16-bit column offsets; a global sort
+    (sigma >= N) scatters rows, offsets exceed int16, and they keep int32
+    columns.  Both reach 1e-10 in FP64 and BF16-refined CG."""
+    A = matgen.stencil27(40)
+    S = oracle.crs_to_scs(A.row_ptr, A.col, A.val, 64, sigma)
+    rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
+    spread = np.abs(S["perm"][A.col].astype(np.int64) - S["perm"][rows]).max()
+    assert (spread <= 32767) == (sigma == 1)
+    for mixed in (False, True):
+        rep, b, x = solve(ctx, A, 64, sigma, mixed=mixed, inner=1e-1)
+        assert rep.status == capi.SUCCESS
+        assert true_rel_residual(A, b, x) <= 1e-10
+
+
 @pytest.mark.parametrize("restart", [0, 1])
 @pytest.mark.parametrize("inner", [1e-1, 1e-4])
 @pytest.mark.parametrize("lo,diag", [(capi.F32, False), (capi.BF16, True),
