@@ -251,19 +251,56 @@ def mtx_section(ctx, path, steps, warmup) -> dict:
     return out
 
 
-def vendor_section(ctx, A, rp, col, val, dx, steps, warmup) -> dict:
-    """rocSPARSE CSR SpMV (FP64) on the same device arrays: a library
-    baseline for comparison only (smaxkernels_amd/vendor/rocsparse_cmp.cpp)."""
+def vendor_lib():
+    """The rocSPARSE comparison library (smaxkernels_amd/vendor/
+    rocsparse_cmp.cpp, benchmark only), or an error string."""
     import ctypes as C
     path = ROOT / "smaxkernels_amd" / "_build" / "librocsparse_cmp.so"
     try:
         lib = C.CDLL(str(path))
     except OSError as e:
-        return {"error": f"rocSPARSE comparison not loaded: {e}"}
+        return f"rocSPARSE comparison not loaded: {e}"
     lib.rocsparse_cmp_create.restype = C.c_void_p
     lib.rocsparse_cmp_create.argtypes = [C.c_int] + [C.c_int64] * 3 + [C.c_void_p] * 6
     lib.rocsparse_cmp_spmv.argtypes = [C.c_void_p]
     lib.rocsparse_cmp_destroy.argtypes = [C.c_void_p]
+    lib.rocsparse_cmp_spmm_create.restype = C.c_void_p
+    lib.rocsparse_cmp_spmm_create.argtypes = ([C.c_int] + [C.c_int64] * 4 + [C.c_void_p] * 3
+                                              + [C.c_int64] + [C.c_void_p] * 3)
+    lib.rocsparse_cmp_spmm.argtypes = [C.c_void_p]
+    lib.rocsparse_cmp_spmm_destroy.argtypes = [C.c_void_p]
+    return lib
+
+
+def vendor_spmm(ctx, lib, algs, block_dim, n_rows, n_cols, nnz, rp, col, val, k, X, Y,
+                ours, steps, warmup, balg, flops):
+    """rocSPARSE FP32 SpMM (CSR or BSR) on the same arrays; per algorithm the
+    time and the largest difference from our result `ours`."""
+    stream = ctx.lib.smax_hip_get_stream(ctx.handle)
+    out = {}
+    for alg, name in algs:
+        h = lib.rocsparse_cmp_spmm_create(alg, block_dim, n_rows, n_cols, nnz, capi._ptr(rp),
+                                          capi._ptr(col), capi._ptr(val), k, capi._ptr(X),
+                                          capi._ptr(Y), stream)
+        if not h:
+            out[name] = None
+            continue
+        ms = time_launches(ctx, lambda: lib.rocsparse_cmp_spmm(h), steps, warmup)
+        got = ctx.download(Y)
+        out[name] = {"ms": round(ms, 4), "GBs_alg": round(balg / ms / 1e6, 1),
+                     "GFLOPs": round(flops / ms / 1e6, 1),
+                     "max_rel_diff_vs_ours":
+                         float(np.max(np.abs(got - ours)) / np.max(np.abs(ours)))}
+        lib.rocsparse_cmp_spmm_destroy(h)
+    return out
+
+
+def vendor_section(ctx, A, rp, col, val, dx, steps, warmup) -> dict:
+    """rocSPARSE CSR SpMV (FP64) on the same device arrays: a library
+    baseline for comparison only (smaxkernels_amd/vendor/rocsparse_cmp.cpp)."""
+    lib = vendor_lib()
+    if isinstance(lib, str):
+        return {"error": lib}
     stream = ctx.lib.smax_hip_get_stream(ctx.handle)
     y = ctx.zeros(A.n_rows, np.float64)
     ctx.crs_spmv(capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, val, dx, y, None)
@@ -332,9 +369,12 @@ def c4_section(ctx, steps, warmup) -> dict:
     T = ctx.scs_transpose_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 4, 64,
                                    A.n_cols)
     ba = A.nnz * 8 + (A.n_rows + 1) * 4 + (A.n_rows + A.n_cols) * k * 4
+    plan = ctx.crs_plan(A.n_rows, A.nnz, rp)
     runs = {
         "forward_crs": lambda: ctx.crs_spmm(capi.F32, A.n_rows, A.n_cols, A.nnz, rp, col,
                                             val, k, True, dX, k, dY, k),
+        "forward_crs_plan": lambda: ctx.crs_spmm(capi.F32, A.n_rows, A.n_cols, A.nnz, rp,
+                                                 col, val, k, True, dX, k, dY, k, plan),
         "forward_sell": lambda: S.apply_spmm(capi.F32, k, dX, dY),
         "adjoint_sell_AT": lambda: T.apply_spmm(capi.F32, k, dW, dG),
         "sddmm_sell": lambda: S.sddmm(capi.F32, k, dW, dX, ddv),
@@ -344,7 +384,17 @@ def c4_section(ctx, steps, warmup) -> dict:
         ms = time_launches(ctx, fn, steps, warmup)
         out[name] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1),
                      "GFLOPs": round(2 * A.nnz * k / ms / 1e6, 1)}
+    lib = vendor_lib()
+    if not isinstance(lib, str):
+        runs["forward_crs"]()
+        ours = ctx.download(dY)
+        out["rocsparse_csr_spmm"] = vendor_spmm(
+            ctx, lib, ((1, "csr"), (2, "csr_row_split"), (3, "csr_nnz_split"),
+                       (4, "csr_merge_path")),
+            0, A.n_rows, A.n_cols, A.nnz, rp, col, val, k, dX, dY, ours, steps, warmup, ba,
+            2 * A.nnz * k)
     log(f"C4 {out}")
+    ctx.crs_plan_destroy(plan)
     for d in (rp, col, val, dX, dW, dY, dG, ddv):
         d.free()
     S.destroy()
@@ -376,9 +426,32 @@ def c5_section(ctx, steps, warmup) -> dict:
                                    "B_alg": ba}
         X.free()
         Y.free()
-    log(f"C5 {out}")
     B.destroy()
     T.destroy()
+    # rocSPARSE has no BF16-value x FP32-vector SpMM: compare both in FP32
+    lib = vendor_lib()
+    if not isinstance(lib, str):
+        rp, col = ctx.upload(A.row_ptr), ctx.upload(A.col)
+        v32 = ctx.upload(A.val.astype(np.float32))
+        B = ctx.bcrs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v32, 4, 4, 4, 4, 4, False)
+        for d in (rp, col, v32):
+            d.free()
+        i = B.info
+        for k in (8, 64):
+            ba = nb * (16 * 4 + 4) + (i.n_block_rows + 1) * 4 + 2 * A.n_rows * k * 4
+            X = ctx.upload(matgen.uniform_f32(matgen.SEED_X, A.n_rows * k))
+            Y = ctx.zeros(A.n_rows * k, np.float32)
+            ms = time_launches(ctx, lambda: B.spmm(capi.F32, k, X, Y), steps, warmup)
+            ours = ctx.download(Y)
+            out[f"f32_forward_k{k}"] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1),
+                                        "GFLOPs": round(2 * nb * 16 * k / ms / 1e6, 1)}
+            out[f"rocsparse_bsr_f32_k{k}"] = vendor_spmm(
+                ctx, lib, ((5, "bsr"),), 4, i.n_block_rows, i.n_block_cols, i.n_blocks,
+                i.row_ptr, i.col, i.val, k, X, Y, ours, steps, warmup, ba, 2 * nb * 16 * k)
+            X.free()
+            Y.free()
+        B.destroy()
+    log(f"C5 {out}")
     return out
 
 

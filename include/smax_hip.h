@@ -138,10 +138,10 @@ int smax_hip_crs_spmv(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                       const int32_t *col, const void *val, const void *x,
                       void *y, smax_hip_crs_plan plan);
 
-/* Row-block plan (SMAX_HIP_CRS_BLOCKED) of one CRS pattern (device
- * row_ptr), built once on the GPU and reused by every smax_hip_crs_spmv on
- * that pattern.  No reference counterpart: the reference CUDA kernel is
- * thread-per-row only. */
+/* Plan of one CRS pattern (device row_ptr), built once and reused by every
+ * smax_hip_crs_spmv (row blocks, SMAX_HIP_CRS_BLOCKED) and smax_hip_crs_spmm
+ * (row execution order) on that pattern.  No reference counterpart: the
+ * reference CUDA kernel is thread-per-row only. */
 int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
                              const int32_t *row_ptr, smax_hip_crs_plan *out);
 int smax_hip_crs_plan_destroy(smax_hip_ctx ctx, smax_hip_crs_plan plan);
@@ -154,13 +154,16 @@ int smax_hip_crs_plan_destroy(smax_hip_ctx ctx, smax_hip_crs_plan plan);
  * Replaces naive_crs_spmm_{col,row}_maj
  * (src/kernels/spmm/spmm_cpu/spmm_cpu_crs_impl.hpp:9-89), which has no CUDA
  * counterpart (src/kernels/spmm.hpp:75-84).  Each output is a sequential fma
- * chain in CRS order (bit-identical to the reference CPU kernel).
+ * chain in CRS order (bit-identical to the reference CPU kernel).  A plan of
+ * the same pattern (smax_hip_crs_plan_create; may be NULL) makes row-major
+ * products run long rows first and the rest by length inside 65536-row
+ * windows; results are the same with or without it.
  */
 int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                       int64_t n_cols, int64_t nnz, const int32_t *row_ptr,
                       const int32_t *col, const void *val, int64_t k,
                       int row_major, const void *X, int64_t ldx, void *Y,
-                      int64_t ldy);
+                      int64_t ldy, smax_hip_crs_plan plan);
 
 /* ------------------------------------------------------------ SELL-C-sigma */
 /*

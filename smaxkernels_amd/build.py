@@ -107,7 +107,7 @@ def build_interface() -> Path:
 
 
 def build_vendor_cmp() -> Path | None:
-    """rocSPARSE CSR SpMV wrapper used by bench.py as a library baseline
+    """rocSPARSE CSR SpMV, CSR SpMM and BSR SpMM wrappers used by bench.py as a library baseline
     (comparison only); skipped when rocSPARSE is not installed."""
     rocm = Path(HIPCC).resolve().parent.parent
     if not (rocm / "lib" / "librocsparse.so").exists():

@@ -89,7 +89,7 @@ _SIGS = {
     "smax_hip_crs_plan_create": ([vp, i64, i64, vp, C.POINTER(vp)], C.c_int),
     "smax_hip_crs_plan_destroy": ([vp, vp], C.c_int),
     "smax_hip_crs_spmm": ([vp, C.c_int, i64, i64, i64, vp, vp, vp, i64, C.c_int,
-                           vp, i64, vp, i64], C.c_int),
+                           vp, i64, vp, i64, vp], C.c_int),
     "smax_hip_scs_spmv": ([vp, C.c_int, i64, i64, vp, vp, vp, vp, vp, vp],
                           C.c_int),
     "smax_hip_scs_spmv_unpermuted": ([vp, C.c_int, i64, i64, i64, vp, vp, vp,
@@ -296,10 +296,10 @@ class Context:
         self._check(self.lib.smax_hip_crs_plan_destroy(self.handle, plan))
 
     def crs_spmm(self, vtype, n_rows, n_cols, nnz, row_ptr, col, val, k,
-                 row_major, X, ldx, Y, ldy) -> None:
+                 row_major, X, ldx, Y, ldy, plan=None) -> None:
         self._check(self.lib.smax_hip_crs_spmm(
             self.handle, vtype, n_rows, n_cols, nnz, _ptr(row_ptr), _ptr(col),
-            _ptr(val), k, int(row_major), _ptr(X), ldx, _ptr(Y), ldy))
+            _ptr(val), k, int(row_major), _ptr(X), ldx, _ptr(Y), ldy, plan))
 
     def scs_spmv(self, vtype, C_, n_chunks, chunk_ptr, chunk_len, col, val, x,
                  y) -> None:

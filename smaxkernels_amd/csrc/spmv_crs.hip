@@ -16,12 +16,17 @@
 //    fully coalesced; a long row is streamed by its whole workgroup.  Row
 //    blocks are dealt to the XCDs in contiguous ranges (xcd_block), so the x
 //    band of neighbouring blocks is shared in one L2.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "common.hpp"
 #include "scan.hpp"
 
 struct smax_hip_crs_plan_s {
     int64_t n_rows = 0, nnz = 0, n_blocks = 0;
     int32_t *block_start = nullptr;  // n_blocks + 1 row indices
+    int32_t *row_order = nullptr;    // row-major SpMM execution order (n_rows)
 };
 
 namespace smaxhip {
@@ -149,8 +154,8 @@ __global__ void crs_block_starts(int64_t n_rows,
 // column-major ones (Y columns are written contiguously).
 template <int VT, bool ROW_MAJOR>
 __global__ __launch_bounds__(kCrsThreads) void crs_spmm_kernel(
-    int64_t n_rows, int64_t k, const int32_t *__restrict__ row_ptr,
-    const int32_t *__restrict__ col,
+    int64_t n_rows, int64_t k, const int32_t *__restrict__ order,
+    const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
     const typename vt<VT>::val_t *__restrict__ val,
     const typename vt<VT>::vec_t *__restrict__ X, int64_t ldx,
     typename vt<VT>::vec_t *__restrict__ Y, int64_t ldy) {
@@ -165,6 +170,7 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_kernel(
         r = t - v * n_rows;
     }
     if (r >= n_rows || v >= k) return;
+    if (ROW_MAJOR && order) r = order[r];
     const int64_t s = row_ptr[r], e = row_ptr[r + 1];
     typename T::vec_t acc = 0;
     int64_t j = s;
@@ -201,7 +207,8 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_kernel(
 // same sequential chain over the row as in crs_spmm_kernel.
 template <int VT, int K>
 __global__ __launch_bounds__(kCrsThreads) void crs_spmm_rowvec(
-    int64_t n_rows, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+    int64_t n_rows, const int32_t *__restrict__ order,
+    const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
     const typename vt<VT>::val_t *__restrict__ val,
     const typename vt<VT>::vec_t *__restrict__ X, int64_t ldx,
     typename vt<VT>::vec_t *__restrict__ Y, int64_t ldy) {
@@ -213,9 +220,10 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_rowvec(
     constexpr int U = 8;
     static_assert(GL * Q == K, "K must be a multiple of the run length");
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kCrsThreads + threadIdx.x;
-    const int64_t r = t / GL;
+    int64_t r = t / GL;
     const int l = static_cast<int>(t % GL);
     if (r >= n_rows) return;
+    if (order) r = order[r];
     const int64_t s = row_ptr[r], e = row_ptr[r + 1];
     A acc[Q];
 #pragma unroll
@@ -271,7 +279,8 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_rowvec(
 // their row strides are 16-byte aligned; false otherwise.
 template <int VT>
 static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
-                                   const int32_t *row_ptr, const int32_t *col,
+                                   const int32_t *order, const int32_t *row_ptr,
+                                   const int32_t *col,
                                    const void *val, const void *X, int64_t ldx, void *Y,
                                    int64_t ldy) {
     using T = vt<VT>;
@@ -289,7 +298,7 @@ static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
     case KK:                                                                                  \
         if constexpr (KK >= Q)                                                                \
             hipLaunchKernelGGL((crs_spmm_rowvec<VT, KK>), g, dim3(kCrsThreads), 0, ctx->stream, \
-                               n_rows, row_ptr, col, v, xv, ldx, yv, ldy);                    \
+                               n_rows, order, row_ptr, col, v, xv, ldx, yv, ldy);             \
         return true;
     switch (k) {
         SMAX_CRS_ROWVEC(2)
@@ -302,6 +311,47 @@ static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
         return false;
     }
 #undef SMAX_CRS_ROWVEC
+}
+
+// Row-major SpMM execution order: every thread group runs one row's chains,
+// so a wave lasts as long as its longest row.  Rows of at least
+// max(128, 8 x mean) entries start first, longest first (they set the
+// critical path); the rest follow window by window (kSpmmWindow rows, so a
+// window's X band stays together), longest first inside each window so that
+// the rows sharing a wave have similar lengths.  Results do not depend on
+// the order.
+constexpr int64_t kSpmmWindow = 65536;
+
+static int build_row_order(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
+                           const int32_t *row_ptr, int32_t **out) {
+    *out = nullptr;
+    std::vector<int32_t> rp(static_cast<size_t>(n_rows + 1));
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(rp.data(), row_ptr, sizeof(int32_t) * (n_rows + 1),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    const double mean = static_cast<double>(nnz) / static_cast<double>(n_rows);
+    const int64_t thr =
+        std::max<int64_t>(128, static_cast<int64_t>(std::ceil(8 * mean)));
+    auto len = [&](int32_t r) { return rp[r + 1] - rp[r]; };
+    std::vector<int32_t> ord;
+    ord.reserve(static_cast<size_t>(n_rows));
+    for (int64_t r = 0; r < n_rows; ++r)
+        if (len(static_cast<int32_t>(r)) >= thr) ord.push_back(static_cast<int32_t>(r));
+    auto longer = [&](int32_t a, int32_t b) { return len(a) > len(b); };
+    std::stable_sort(ord.begin(), ord.end(), longer);
+    for (int64_t w = 0; w < n_rows; w += kSpmmWindow) {
+        const size_t first = ord.size();
+        for (int64_t r = w; r < std::min(n_rows, w + kSpmmWindow); ++r)
+            if (len(static_cast<int32_t>(r)) < thr) ord.push_back(static_cast<int32_t>(r));
+        std::stable_sort(ord.begin() + static_cast<std::ptrdiff_t>(first), ord.end(), longer);
+    }
+    int rc = dev_alloc(ctx, reinterpret_cast<void **>(out), sizeof(int32_t) * n_rows,
+                       "crs plan row order");
+    if (rc) return rc;
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(*out, ord.data(), sizeof(int32_t) * n_rows,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    return SMAX_HIP_SUCCESS;
 }
 
 }  // namespace smaxhip
@@ -363,8 +413,11 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     }
     dev_free(flag);
     dev_free(tmp);
+    int32_t *order = nullptr;
+    if (!rc) rc = build_row_order(ctx, n_rows, nnz, row_ptr, &order);
     if (rc) {
         dev_free(starts);
+        dev_free(order);
         return rc;
     }
     smax_hip_crs_plan p = new smax_hip_crs_plan_s();
@@ -372,6 +425,7 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     p->nnz = nnz;
     p->n_blocks = n_blocks;
     p->block_start = starts;
+    p->row_order = order;
     *out = p;
     return SMAX_HIP_SUCCESS;
 }
@@ -383,6 +437,7 @@ int smax_hip_crs_plan_destroy(smax_hip_ctx ctx, smax_hip_crs_plan plan) {
         (void)hipStreamSynchronize(ctx->stream);
     }
     dev_free(plan->block_start);
+    dev_free(plan->row_order);
     delete plan;
     return SMAX_HIP_SUCCESS;
 }
@@ -429,7 +484,7 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                       int64_t n_cols, int64_t nnz, const int32_t *row_ptr,
                       const int32_t *col, const void *val, int64_t k,
                       int row_major, const void *X, int64_t ldx, void *Y,
-                      int64_t ldy) {
+                      int64_t ldy, smax_hip_crs_plan plan) {
     SMAX_HIP_CHECK_CTX(ctx);
     if (!valid_value_type(value_type))
         return fail(ctx, SMAX_HIP_ERROR_UNSUPPORTED,
@@ -444,26 +499,30 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
     else
         SMAX_HIP_REQUIRE(ctx, ldx >= n_cols || ldx >= n_rows,
                          "smax_hip_crs_spmm: column-major ldx too small");
+    if (plan)
+        SMAX_HIP_REQUIRE(ctx, plan->n_rows == n_rows && plan->nnz == nnz,
+                         "smax_hip_crs_spmm: plan built for another pattern");
+    const int32_t *order = plan ? plan->row_order : nullptr;
     const int64_t threads = n_rows * k;
     SMAX_HIP_DISPATCH_VT(value_type, {
         using T = vt<VT>;
         auto v = static_cast<const typename T::val_t *>(val);
         auto xv = static_cast<const typename T::vec_t *>(X);
         auto yv = static_cast<typename T::vec_t *>(Y);
-        if (row_major && launch_crs_spmm_rowvec<VT>(ctx, n_rows, k, row_ptr, col, val, X, ldx,
-                                                    Y, ldy)) {
+        if (row_major && launch_crs_spmm_rowvec<VT>(ctx, n_rows, k, order, row_ptr, col, val,
+                                                    X, ldx, Y, ldy)) {
             SMAX_HIP_LAUNCHED(ctx, "crs_spmm_rowvec");
             return SMAX_HIP_SUCCESS;
         }
         if (row_major)
             hipLaunchKernelGGL((crs_spmm_kernel<VT, true>),
                                grid_for(threads, kCrsThreads), dim3(kCrsThreads),
-                               0, ctx->stream, n_rows, k, row_ptr, col, v, xv,
+                               0, ctx->stream, n_rows, k, order, row_ptr, col, v, xv,
                                ldx, yv, ldy);
         else
             hipLaunchKernelGGL((crs_spmm_kernel<VT, false>),
                                grid_for(threads, kCrsThreads), dim3(kCrsThreads),
-                               0, ctx->stream, n_rows, k, row_ptr, col, v, xv,
+                               0, ctx->stream, n_rows, k, nullptr, row_ptr, col, v, xv,
                                ldx, yv, ldy);
         SMAX_HIP_LAUNCHED(ctx, "crs_spmm_kernel");
         return SMAX_HIP_SUCCESS;

@@ -27,8 +27,15 @@ class SpMMKernel : public Kernel {
     SPMM::Flags flags;
     PLATFORM::HIP::Stream stream;
     PLATFORM::HIP::DeviceBuffer row_ptr, col, val, X, Y;
+    // row execution order for row-major products, keyed by the host row_ptr
+    smax_hip_crs_plan plan = nullptr;
+    const void *plan_row_ptr = nullptr;
+    ULL plan_rows = 0, plan_nnz = 0;
 
     explicit SpMMKernel(std::unique_ptr<KernelContext> ctx) : Kernel(std::move(ctx)) {}
+    ~SpMMKernel() override {
+        if (plan) smax_hip_crs_plan_destroy(stream.get(), plan);
+    }
 
     int _register_A(const std::vector<Variant> &a) override {
         if (a.size() != 6) throw std::runtime_error("SpMMKernel register_A expects 6 args");
@@ -76,6 +83,17 @@ class SpMMKernel : public Kernel {
         rc |= val.upload(c, A.val, vb * A.nnz);
         rc |= X.upload(c, args.X.val, xb * args.X.n_rows * args.X.n_cols);
         rc |= Y.upload(c, args.Y.val, xb * args.Y.n_rows * args.Y.n_cols);
+        if (!rc && flags.vec_row_major &&
+            (!plan || plan_row_ptr != A.row_ptr || plan_rows != A.n_rows || plan_nnz != A.nnz)) {
+            if (plan) smax_hip_crs_plan_destroy(c, plan);
+            plan = nullptr;
+            rc |= PLATFORM::HIP::report(
+                c, smax_hip_crs_plan_create(c, A.n_rows, A.nnz, row_ptr.as<int32_t>(), &plan),
+                "CRS SpMM plan");
+            plan_row_ptr = A.row_ptr;
+            plan_rows = A.n_rows;
+            plan_nnz = A.nnz;
+        }
         timers->get("initialize")->stop();
         return rc ? 1 : 0;
     }
@@ -95,7 +113,8 @@ class SpMMKernel : public Kernel {
             A.n_cols, A.nnz, row_ptr.as<int32_t>(), col.as<int32_t>(), val.get(), k,
             flags.vec_row_major ? 1 : 0,
             static_cast<const char *>(X.get()) + xb * X_offset, ld,
-            static_cast<char *>(Y.get()) + xb * Y_offset, ld);
+            static_cast<char *>(Y.get()) + xb * Y_offset, ld,
+            flags.vec_row_major ? plan : nullptr);
         timers->get("apply")->stop();
         return PLATFORM::HIP::report(stream.get(), st, "SpMM launch");
     }

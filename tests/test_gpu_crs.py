@@ -106,17 +106,25 @@ def test_crs_spmv_long_and_empty_rows(ctx, vt):
 @pytest.mark.parametrize("vt", VTS)
 @pytest.mark.parametrize("k", [1, 4, 8, 32])
 @pytest.mark.parametrize("row_major", [True, False])
-def test_crs_spmm_bit_exact(ctx, vt, k, row_major):
+@pytest.mark.parametrize("planned", [False, True])
+def test_crs_spmm_bit_exact(ctx, vt, k, row_major, planned):
+    """With a plan, row-major products run in the plan's row order (long rows
+    first, then by length per window): the same chains, the same Y."""
     A = matgen.lomax_band(20000, half=2500)
     val, X = operands(vt, A, k)
     rp, col, v, dX = (ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(val),
                       ctx.upload(X))
     dY = ctx.zeros(A.n_rows * k, X.dtype)
     ld = k if row_major else A.n_rows
+    plan = ctx.crs_plan(A.n_rows, A.nnz, rp) if planned else None
     ctx.crs_spmm(vt, A.n_rows, A.n_cols, A.nnz, rp, col, v, k, row_major, dX, ld,
-                 dY, ld)
+                 dY, ld, plan)
     assert_bitwise(ctx.download(dY),
                    oracle.crs_spmm(A.row_ptr, A.col, val, k, row_major, X))
+    if plan:
+        ctx.crs_plan_destroy(plan)
+    for d in (rp, col, v, dX, dY):
+        d.free()
 
 
 @pytest.mark.parametrize("vt", VTS)
