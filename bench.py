@@ -568,32 +568,43 @@ def main() -> int:
         prp, pcol, pval, pdx = (ctx.upload(P.row_ptr), ctx.upload(P.col),
                                 ctx.upload(P.val), ctx.upload(px))
         ps = ctx.scs_from_crs(P.n_rows, P.n_cols, P.nnz, prp, pcol, pval, 8, 64, 1)
+        pplan = ctx.crs_plan(P.n_rows, P.nnz, prp)
         pdy = ctx.zeros(ps.info.n_rows_padded, np.float64)
-        warm_py = time_launches(ctx, lambda: ps.apply(capi.F64, pdx, pdy), 2000, 100)
-        warm = graph_ms(ctx, lambda: ps.apply(capi.F64, pdx, pdy),
-                        check=lambda: ctx.download(pdy))
         scrub_n = 1 << 27  # 1 GiB read + 1 GiB written between launches
         s1, s2 = ctx.zeros(scrub_n, np.float64), ctx.zeros(scrub_n, np.float64)
-        cold = []
-        for _ in range(20):
-            ctx.stream_copy(s1, s2, scrub_n)
-            ctx.timer_start()
-            ps.apply(capi.F64, pdx, pdy)
-            cold.append(ctx.timer_stop())
-        cold_ms = float(np.median(cold))
-        extra["c1_poisson512_f64_sell64_1"] = {
-            "B_alg": pb, "warm_ms": round(warm, 5),
-            "warm_GBs": round(pb / warm / 1e6, 1),
-            "warm_method": "HIP graph of 200 back-to-back SpMVs, 10 replays",
-            "warm_frac_of_measured_stream": (round(pb / warm / 1e6 / peak_gbs, 3)
-                                             if peak_gbs else None),
-            "warm_host_launch_ms": round(warm_py, 5),
-            "cold_ms": round(cold_ms, 5), "cold_GBs": round(pb / cold_ms / 1e6, 1),
-            "cold_method": "1 GiB copy between launches, median of 20 event-timed"}
+
+        def c1_times(fn) -> dict:
+            warm_py = time_launches(ctx, fn, 2000, 100)
+            warm = graph_ms(ctx, fn, check=lambda: ctx.download(pdy))
+            cold = []
+            for _ in range(20):
+                ctx.stream_copy(s1, s2, scrub_n)
+                ctx.timer_start()
+                fn()
+                cold.append(ctx.timer_stop())
+            cold_ms = float(np.median(cold))
+            return {
+                "B_alg": pb, "warm_ms": round(warm, 5),
+                "warm_GBs": round(pb / warm / 1e6, 1),
+                "warm_method": "HIP graph of 200 back-to-back SpMVs, 10 replays",
+                "warm_frac_of_measured_stream": (round(pb / warm / 1e6 / peak_gbs, 3)
+                                                 if peak_gbs else None),
+                "warm_host_launch_ms": round(warm_py, 5),
+                "cold_ms": round(cold_ms, 5), "cold_GBs": round(pb / cold_ms / 1e6, 1),
+                "cold_method": "1 GiB copy between launches, median of 20 event-timed"}
+
+        extra["c1_poisson512_f64_sell64_1"] = c1_times(lambda: ps.apply(capi.F64, pdx, pdy))
+        # config 1 is the reference's plain CRS SpMV: both CRS schedules too
+        extra["c1_poisson512_f64_crs"] = {
+            "lane_per_row": c1_times(lambda: ctx.crs_spmv(capi.F64, P.n_rows, P.n_cols, P.nnz,
+                                                          prp, pcol, pval, pdx, pdy, None)),
+            "row_blocks": c1_times(lambda: ctx.crs_spmv(capi.F64, P.n_rows, P.n_cols, P.nnz,
+                                                        prp, pcol, pval, pdx, pdy, pplan))}
+        ctx.crs_plan_destroy(pplan)
         for d in (s1, s2, pdy, prp, pcol, pval, pdx):
             d.free()
         ps.destroy()
-        log(f"C1 {extra['c1_poisson512_f64_sell64_1']}")
+        log(f"C1 {extra['c1_poisson512_f64_sell64_1']} {extra['c1_poisson512_f64_crs']}")
 
     if args.mtx:
         extra["mtx"] = mtx_section(ctx, args.mtx, args.steps, args.warmup)

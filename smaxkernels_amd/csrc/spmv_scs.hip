@@ -20,44 +20,14 @@
 namespace smaxhip {
 
 constexpr int kScsThreads = 256;
-constexpr int kScsUnroll = 8;
-constexpr int kScsPipeUnroll = 16;
+constexpr int kScsUnroll = 8;       // generic-C chains
+constexpr int kScsPipeUnroll = 16;  // C = 64 chains
 
-template <int VT, int U>
-__device__ __forceinline__ typename vt<VT>::vec_t
-slot_chain(const int32_t *__restrict__ col,
-           const typename vt<VT>::val_t *__restrict__ val,
-           const typename vt<VT>::vec_t *__restrict__ x, int64_t base,
-           int64_t stride, int len) {
-    using T = vt<VT>;
-    using A = typename T::vec_t;
-    A acc = 0;
-    int j = 0;
-    for (; j + U <= len; j += U) {
-        int32_t c[U];
-        typename T::val_t v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            c[u] = ld_stream(col + base + (j + u) * stride);
-            v[u] = ld_stream(val + base + (j + u) * stride);
-        }
-        A xv[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
-#pragma unroll
-        for (int u = 0; u < U; ++u) acc = T::fma_(T::widen(v[u]), xv[u], acc);
-    }
-    for (; j < len; ++j) {
-        const int32_t c = ld_stream(col + base + j * stride);
-        const auto v = ld_stream(val + base + j * stride);
-        acc = T::fma_(T::widen(v), x[c], acc);
-    }
-    return acc;
-}
-
-// As slot_chain, software-pipelined: the next group's column indices and
-// values are requested before this group's x gathers, so a long chain waits
-// about one memory round trip per group instead of two.
+// One slot's chain in groups of U: the next group's column indices and
+// values are requested before this group's x gathers (software pipelining),
+// so a long chain waits about one memory round trip per group instead of
+// two.  The remainder runs element by element: a predicated group for it
+// costs registers and measured slower on the 4M-row band matrix.
 template <int VT, int U>
 __device__ __forceinline__ typename vt<VT>::vec_t
 slot_chain_pipe(const int32_t *__restrict__ col,
