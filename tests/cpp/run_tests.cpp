@@ -3,7 +3,8 @@
 //   run_tests host   API behaviour that needs no GPU (registration, argument
 //                    checks, refused platforms and types)
 //   run_tests gpu    the reference's SpMV / SELL / BCRS / SpMM known answers
-//                    (examples/tests/*.hpp) run through PlatformType::HIP
+//                    (examples/tests/*.hpp) run through PlatformType::HIP,
+//                    and the Jacobi application loop against a host Jacobi
 //
 // Exit status 0 when every selected test passes.
 #include <cmath>
@@ -335,6 +336,78 @@ TEST(gpu, jacobi_style_swap_and_offsets) {
     // (spmv_cuda_core.cuh:190-200): y[1..3] = A * (1,2,3), y[0] untouched
     const double w1 = std::fma(1.2, 2.0, 1.1 * 1.0);
     close(&yl[1], &w1, 1, "offset y[1]");
+}
+
+TEST(gpu, jacobi_poisson_matches_host_iterates) {
+    // The reference's Jacobi application loop (examples/applications/jacobi/
+    // jacobi.cpp:26-58): x_new <- A x_old through run(), the host normalises
+    // x_new = (b - (A x_old - D x_old)) / D, then swap_operands and the
+    // application's pointer swap.  Every run re-uploads the swapped
+    // operands.  The GPU SpMV is the reference fma chain, so the iterates
+    // equal a host Jacobi with the same chain bit for bit, iteration by
+    // iteration, until the residual test stops both.
+    const int n = 24, N = n * n;
+    std::vector<int> rp(1, 0), cl;
+    std::vector<double> va;
+    for (int r = 0; r < N; ++r) {
+        const int i = r / n, j = r % n;
+        if (i > 0) { cl.push_back(r - n); va.push_back(-1.0); }
+        if (j > 0) { cl.push_back(r - 1); va.push_back(-1.0); }
+        cl.push_back(r); va.push_back(4.0);
+        if (j < n - 1) { cl.push_back(r + 1); va.push_back(-1.0); }
+        if (i < n - 1) { cl.push_back(r + n); va.push_back(-1.0); }
+        rp.push_back(static_cast<int>(cl.size()));
+    }
+    const std::vector<double> b(N, 1.0), D(N, 4.0);
+    auto spmv_host = [&](const std::vector<double> &x, std::vector<double> &y) {
+        for (int r = 0; r < N; ++r) {
+            double s = 0;
+            for (int e = rp[r]; e < rp[r + 1]; ++e) s = std::fma(va[e], x[cl[e]], s);
+            y[r] = s;
+        }
+    };
+    auto normalize = [&](std::vector<double> &xn, const std::vector<double> &xo) {
+        for (int r = 0; r < N; ++r) xn[r] = (b[r] - (xn[r] - D[r] * xo[r])) / D[r];
+    };
+    auto residual = [&](const std::vector<double> &x) {
+        std::vector<double> t(N);
+        spmv_host(x, t);
+        double m = 0;
+        for (int r = 0; r < N; ++r) m = std::max(m, std::fabs(b[r] - t[r]));
+        return m;
+    };
+    const double tol = 1e-6;
+    const int max_iters = 5000, check_every = 10;
+    // host Jacobi
+    std::vector<double> ho(N, 0.0), hn(N, 0.0);
+    int host_iters = 0;
+    for (double res = residual(ho); res > tol && host_iters < max_iters;) {
+        spmv_host(ho, hn);
+        normalize(hn, ho);
+        std::swap(ho, hn);
+        if (++host_iters % check_every == 0) res = residual(ho);
+    }
+    // the same loop with the product on the GPU through SMAX::Interface
+    std::vector<double> a(N, 0.0), c(N, 0.0);
+    std::vector<double> *x_old = &a, *x_new = &c;
+    SMAX::Interface smax;
+    smax.register_kernel("x_new <- Ax_old", KernelType::SPMV, PlatformType::HIP);
+    auto *k = smax.kernel("x_new <- Ax_old");
+    k->register_A(N, N, static_cast<int>(cl.size()), cl.data(), rp.data(), va.data());
+    k->register_B(N, x_old->data());
+    k->register_C(N, x_new->data());
+    int iters = 0;
+    for (double res = residual(*x_old); res > tol && iters < max_iters;) {
+        check(k->run() == 0, "jacobi run");
+        normalize(*x_new, *x_old);
+        k->swap_operands();
+        std::swap(x_old, x_new);
+        if (++iters % check_every == 0) res = residual(*x_old);
+    }
+    check(iters == host_iters && iters < max_iters,
+          "iteration count " + std::to_string(iters) + " vs host " + std::to_string(host_iters));
+    check(std::memcmp(x_old->data(), ho.data(), sizeof(double) * N) == 0,
+          "GPU-driven iterate differs from the host Jacobi");
 }
 
 int main(int argc, char **argv) {
