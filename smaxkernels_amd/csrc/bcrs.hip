@@ -153,7 +153,47 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_kernel(
 // 16-byte load.  Per output the fma order is still the reference's (block j,
 // then jx); the next block's values and X entries are loaded while this
 // block's fmas run.
-template <int VT, int BH, int BW, int Q>
+// One block's values, widened to the accumulation type.  DENSE: h_pad == BH
+// and w_pad == BW, so the block is BH*BW contiguous values (row- or
+// column-major), read as 16-byte loads when that size is a multiple of 16
+// bytes (val is then 16-byte aligned, checked by the host); BF16 blocks are
+// read as 32-bit words holding two values each.
+template <int VT, int BH, int BW, bool DENSE>
+__device__ __forceinline__ void load_block(const typename vt<VT>::val_t *__restrict__ blk,
+                                           int64_t vs_i, int64_t vs_j, bool col_major,
+                                           typename vt<VT>::vec_t (&w)[BH][BW]) {
+    using T = vt<VT>;
+    using A = typename T::vec_t;
+    if constexpr (DENSE) {
+        A flat[BH * BW];
+        if constexpr (VT == SMAX_HIP_BF16 && (BH * BW) % 2 == 0) {
+            uint32_t words[BH * BW / 2];
+            load_run<uint32_t, BH * BW / 2>(reinterpret_cast<const uint32_t *>(blk), words);
+#pragma unroll
+            for (int i = 0; i < BH * BW / 2; ++i) {
+                flat[2 * i] = __uint_as_float(words[i] << 16);
+                flat[2 * i + 1] = __uint_as_float(words[i] & 0xffff0000u);
+            }
+        } else {
+            typename T::val_t raw[BH * BW];
+            load_run<typename T::val_t, BH * BW>(blk, raw);
+#pragma unroll
+            for (int i = 0; i < BH * BW; ++i) flat[i] = T::widen(raw[i]);
+        }
+#pragma unroll
+        for (int ix = 0; ix < BH; ++ix)
+#pragma unroll
+            for (int jx = 0; jx < BW; ++jx)
+                w[ix][jx] = col_major ? flat[jx * BH + ix] : flat[ix * BW + jx];
+    } else {
+#pragma unroll
+        for (int jx = 0; jx < BW; ++jx)
+#pragma unroll
+            for (int ix = 0; ix < BH; ++ix) w[ix][jx] = T::widen(blk[ix * vs_i + jx * vs_j]);
+    }
+}
+
+template <int VT, int BH, int BW, int Q, bool DENSE>
 __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     int64_t n_block_rows, int h_pad, int w_pad, int col_major, int64_t k,
     const int32_t *__restrict__ col, const int32_t *__restrict__ row_ptr,
@@ -162,7 +202,6 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     typename vt<VT>::vec_t *__restrict__ Y) {
     using T = vt<VT>;
     using A = typename T::vec_t;
-    using V = typename T::val_t;
     const int64_t gl = k / Q;  // lanes per block row
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
     const int64_t r = t / gl, v = (t - r * gl) * Q;
@@ -177,36 +216,28 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     int64_t j = row_ptr[r];
     const int64_t je = row_ptr[r + 1];
     if (j < je) {
-        V w0[BH][BW];
+        A w0[BH][BW];
         A x0[BW][Q];
         {
-            const V *blk = val + j * bs;
             const A *xr = X + static_cast<int64_t>(col[j]) * w_pad * k + v;
 #pragma unroll
-            for (int jx = 0; jx < BW; ++jx) {
-                load_run<A, Q>(xr + jx * k, x0[jx]);
-#pragma unroll
-                for (int ix = 0; ix < BH; ++ix) w0[ix][jx] = blk[ix * vs_i + jx * vs_j];
-            }
+            for (int jx = 0; jx < BW; ++jx) load_run<A, Q>(xr + jx * k, x0[jx]);
+            load_block<VT, BH, BW, DENSE>(val + j * bs, vs_i, vs_j, col_major, w0);
         }
         for (; j + 1 < je; ++j) {
-            V w1[BH][BW];
+            A w1[BH][BW];
             A x1[BW][Q];
-            const V *blk = val + (j + 1) * bs;
             const A *xr = X + static_cast<int64_t>(col[j + 1]) * w_pad * k + v;
 #pragma unroll
-            for (int jx = 0; jx < BW; ++jx) {
-                load_run<A, Q>(xr + jx * k, x1[jx]);
-#pragma unroll
-                for (int ix = 0; ix < BH; ++ix) w1[ix][jx] = blk[ix * vs_i + jx * vs_j];
-            }
+            for (int jx = 0; jx < BW; ++jx) load_run<A, Q>(xr + jx * k, x1[jx]);
+            load_block<VT, BH, BW, DENSE>(val + (j + 1) * bs, vs_i, vs_j, col_major, w1);
 #pragma unroll
             for (int ix = 0; ix < BH; ++ix)
 #pragma unroll
                 for (int jx = 0; jx < BW; ++jx)
 #pragma unroll
                     for (int q = 0; q < Q; ++q)
-                        acc[ix][q] = T::fma_(T::widen(w0[ix][jx]), x0[jx][q], acc[ix][q]);
+                        acc[ix][q] = T::fma_(w0[ix][jx], x0[jx][q], acc[ix][q]);
 #pragma unroll
             for (int jx = 0; jx < BW; ++jx) {
 #pragma unroll
@@ -221,7 +252,7 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
             for (int jx = 0; jx < BW; ++jx)
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
-                    acc[ix][q] = T::fma_(T::widen(w0[ix][jx]), x0[jx][q], acc[ix][q]);
+                    acc[ix][q] = T::fma_(w0[ix][jx], x0[jx][q], acc[ix][q]);
     }
 #pragma unroll
     for (int ix = 0; ix < BH; ++ix) store_run<A, Q>(Y + (r * h_pad + ix) * k + v, acc[ix]);
@@ -240,16 +271,21 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
     auto yv = static_cast<A *>(Y);
     // 16-byte runs need at least 4 lanes per block row: with fewer (k = 8 in
     // FP32) the per-lane chains run longer than the saved instructions pay for
-    if (k % Q == 0 && k >= 4 * Q && aligned16(X, Y))
-        hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW, Q>),
-                           grid_for(n_block_rows * (k / Q), kBThreads), dim3(kBThreads), 0,
-                           ctx->stream, n_block_rows, static_cast<int>(h_pad),
-                           static_cast<int>(w_pad), cm ? 1 : 0, k, col, row_ptr, v, xv, yv);
-    else
-        hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW, 1>), grid_for(n_block_rows * k, kBThreads),
-                           dim3(kBThreads), 0, ctx->stream, n_block_rows,
-                           static_cast<int>(h_pad), static_cast<int>(w_pad), cm ? 1 : 0, k,
-                           col, row_ptr, v, xv, yv);
+    const bool runs = k % Q == 0 && k >= 4 * Q && aligned16(X, Y);
+    // unpadded blocks load as one contiguous run (16-byte loads if the block
+    // size is a multiple of 16 bytes and val is aligned)
+    constexpr bool kBlock16 = (BH * BW * sizeof(typename T::val_t)) % 16 == 0;
+    const bool dense = h_pad == BH && w_pad == BW && (!kBlock16 || aligned16(val, val));
+    const dim3 g = grid_for(n_block_rows * (runs ? k / Q : k), kBThreads);
+#define SMAX_BCRS_REG_LAUNCH(QQ, DD)                                                        \
+    hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW, QQ, DD>), g, dim3(kBThreads), 0,          \
+                       ctx->stream, n_block_rows, static_cast<int>(h_pad),                 \
+                       static_cast<int>(w_pad), cm ? 1 : 0, k, col, row_ptr, v, xv, yv)
+    if (runs && dense) SMAX_BCRS_REG_LAUNCH(Q, true);
+    else if (runs) SMAX_BCRS_REG_LAUNCH(Q, false);
+    else if (dense) SMAX_BCRS_REG_LAUNCH(1, true);
+    else SMAX_BCRS_REG_LAUNCH(1, false);
+#undef SMAX_BCRS_REG_LAUNCH
 }
 
 template <int VT, int BW>

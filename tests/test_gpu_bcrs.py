@@ -74,7 +74,8 @@ def vals_for(vt, v64):
 
 @pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
 @pytest.mark.parametrize("bh,bw,hp,wp,cm", [(4, 4, 4, 4, True), (4, 4, 4, 4, False),
-                                            (4, 4, 5, 6, True), (2, 4, 3, 4, False)])
+                                            (4, 4, 5, 6, True), (2, 4, 3, 4, False),
+                                            (2, 4, 2, 4, True), (2, 2, 2, 2, False)])
 def test_block7_conversion_spmv_spmm(ctx, vt, bh, bw, hp, wp, cm):
     A = matgen.block7(10, 4)  # 4000 rows, 4x4 dense blocks
     val = vals_for(vt, A.val)
