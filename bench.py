@@ -600,6 +600,11 @@ def main() -> int:
                                                           prp, pcol, pval, pdx, pdy, None)),
             "row_blocks": c1_times(lambda: ctx.crs_spmv(capi.F64, P.n_rows, P.n_cols, P.nnz,
                                                         prp, pcol, pval, pdx, pdy, pplan))}
+        # the dispatch floor: an 8-byte fill kernel timed the same two ways
+        # (both C1 timings are within a few microseconds of it)
+        floor = c1_times(lambda: ctx._check(ctx.lib.smax_hip_zero(ctx.handle, pdy.ptr, 8)))
+        extra["c1_dispatch_floor"] = {"warm_ms": floor["warm_ms"], "cold_ms": floor["cold_ms"],
+                                      "kernel": "zero_kernel on 8 bytes"}
         ctx.crs_plan_destroy(pplan)
         for d in (s1, s2, pdy, prp, pcol, pval, pdx):
             d.free()
