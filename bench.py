@@ -384,6 +384,8 @@ def c4_section(ctx, steps, warmup) -> dict:
         ms = time_launches(ctx, fn, steps, warmup)
         out[name] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1),
                      "GFLOPs": round(2 * A.nnz * k / ms / 1e6, 1)}
+    out["fd_check"] = fd_check(A, v32, X, W, k, ctx.download(ddv).astype(np.float64),
+                               ctx.download(dG).astype(np.float64).reshape(A.n_cols, k))
     lib = vendor_lib()
     if not isinstance(lib, str):
         runs["forward_crs"]()
@@ -400,6 +402,54 @@ def c4_section(ctx, steps, warmup) -> dict:
     S.destroy()
     T.destroy()
     return out
+
+
+def fd_check(A, v32, Xf, Wf, k, gv, gx, n_samples=64, seed=2024) -> dict:
+    """FP64 central differences of L = sum(Y o W), Y = A X, for n_samples
+    value entries and n_samples X entries, against the SDDMM (gv, CRS order)
+    and A^T W (gx) gradients.  L is linear in each entry, so the central
+    difference is exact up to rounding; it is evaluated on the rows the entry
+    touches (every other term of L cancels).  Metric per entry:
+    |g - g_FD| / max(|g_FD|, sum of |terms|)."""
+    rng = np.random.default_rng(seed)
+    X = Xf.astype(np.float64).reshape(A.n_cols, k)
+    W = Wf.astype(np.float64).reshape(A.n_rows, k)
+    v64 = v32.astype(np.float64)
+    rows_of = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
+
+    def loss(vals, Xm, rows):
+        tot = 0.0
+        for i in rows:
+            s, e = A.row_ptr[i], A.row_ptr[i + 1]
+            tot += float((vals[s:e] @ Xm[A.col[s:e]]) @ W[i])
+        return tot
+
+    def central(arr, at, rows, vals, Xm):
+        # perturb arr[at] by +-h in place (arr is vals or Xm), then restore
+        keep = arr[at]
+        h = 2.0 ** -10 * max(1.0, abs(keep))
+        arr[at] = keep + h
+        lp = loss(vals, Xm, rows)
+        arr[at] = keep - h
+        lm = loss(vals, Xm, rows)
+        arr[at] = keep
+        return (lp - lm) / (2 * h)
+
+    worst = 0.0
+    for j in rng.choice(A.nnz, n_samples, replace=False):
+        i = rows_of[j]
+        fd = central(v64, j, [i], v64, X)
+        scale = max(abs(fd), float(np.abs(W[i] * X[A.col[j]]).sum()))
+        worst = max(worst, abs(gv[j] - fd) / scale)
+    for idx in rng.choice(A.n_cols * k, n_samples, replace=False):
+        jr, vv = divmod(int(idx), k)
+        mask = A.col == jr
+        rows = np.unique(rows_of[mask])
+        fd = central(X, (jr, vv), rows, v64, X)
+        scale = max(abs(fd), float(np.abs(v64[mask] * W[rows_of[mask], vv]).sum()))
+        worst = max(worst, abs(gx[jr, vv] - fd) / scale)
+    return {"entries": 2 * n_samples, "max_rel_err": float(worst), "bound": 1e-6,
+            "metric": "|g - g_FD| / max(|g_FD|, sum |terms|), FP64 central differences"}
 
 
 def c5_section(ctx, steps, warmup) -> dict:
