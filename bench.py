@@ -525,8 +525,9 @@ def main() -> int:
                          "stream,sell,gather,vendor,lowp,crs,c1,cg,c4,c5,cpu")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(row-block sharding is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only; there is no multi-GPU path (rows "
+              "would shard as contiguous row blocks with one x halo exchange per "
+              "product).", file=sys.stderr)
         return 2
     only = set(filter(None, args.only.split(",")))
     want = (lambda s: not only or s in only)
