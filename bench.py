@@ -178,8 +178,10 @@ def cpu_baseline(A, x, seconds) -> dict:
 def cg_section(ctx) -> dict:
     """27-point stencil on 160^3 (N = 4,096,000, nnz = 109,215,352, diagonal
     26.01): FP64 CG vs iterative refinement with BF16 values (FP32 diagonal
-    side array) to ||b - A x|| / ||b|| <= 1e-10, b uniform [-1, 1)."""
-    from oracle import oracle
+    side array) to ||b - A x|| / ||b|| <= 1e-10, b uniform [-1, 1).  Every
+    solution's residual is recomputed on the host in FP64 (scipy's CSR
+    product, independent of the GPU kernels)."""
+    import scipy.sparse as sp
     log("C3: 27-point stencil 160^3")
     A = matgen.stencil27(160)
     b = matgen.uniform_f64(matgen.SEED_B, A.n_rows)
@@ -188,6 +190,7 @@ def cg_section(ctx) -> dict:
     db = ctx.upload(b)
     out = {"n_rows": A.n_rows, "nnz": A.nnz, "format": "SELL-64-1",
            "fill": round(S.info.n_elements / A.nnz, 4)}
+    Ah = sp.csr_matrix((A.val, A.col, A.row_ptr), shape=(A.n_rows, A.n_cols))
 
     def solve(mixed, lo=capi.BF16, inner=0.1, reps=3):
         best = None
@@ -201,11 +204,11 @@ def cg_section(ctx) -> dict:
                 best = (rep, ctx.download(dx))
             dx.free()
         rep, x = best
-        r = b - oracle.crs_spmv(A.row_ptr, A.col, A.val, x)
+        r = b - Ah @ x
         return {"status": rep.status, "seconds": round(rep.seconds, 5),
                 "iterations": rep.iterations, "outer": rep.outer_iterations,
                 "rel_residual_gpu": rep.rel_residual,
-                "rel_residual_oracle": float(np.linalg.norm(r) / np.linalg.norm(b)),
+                "rel_residual_host_fp64": float(np.linalg.norm(r) / np.linalg.norm(b)),
                 "setup_seconds": round(rep.setup_seconds, 5)}, x
 
     fp64, x64 = solve(False)
