@@ -309,25 +309,34 @@ static void launch_bcrs_spmm(smax_hip_ctx ctx, int64_t threads, int64_t n_block_
 // block-column sequences must be non-decreasing (the reference's fill loop,
 // convert_formats.hpp:222-234, throws otherwise).  Writes them to out (if
 // non-null) and returns their count, or -1 on an out-of-order row.
+//
+// BH > 0 fixes b_h at compile time, so the b_h cursors live in registers;
+// BH == 0 takes b_h (<= 64) at run time.
+template <int BH>
 __device__ int64_t merge_block_row(const int32_t *__restrict__ row_ptr,
                                    const int32_t *__restrict__ col, int64_t br,
                                    int b_h, int b_w, int32_t *out) {
-    constexpr int kMaxBh = 64;
-    int64_t pos[kMaxBh], end[kMaxBh];
-    for (int l = 0; l < b_h; ++l) {
-        pos[l] = row_ptr[br * b_h + l];
-        end[l] = row_ptr[br * b_h + l + 1];
-        for (int64_t j = pos[l] + 1; j < end[l]; ++j)
+    constexpr int kMaxBh = BH > 0 ? BH : 64;
+    constexpr int kUnroll = BH > 0 ? BH : 1;
+    const int bh = BH > 0 ? BH : b_h;
+    int32_t pos[kMaxBh], end[kMaxBh];
+#pragma unroll kUnroll
+    for (int l = 0; l < bh; ++l) {
+        pos[l] = row_ptr[br * bh + l];
+        end[l] = row_ptr[br * bh + l + 1];
+        for (int32_t j = pos[l] + 1; j < end[l]; ++j)
             if (col[j] / b_w < col[j - 1] / b_w) return -1;
     }
     int64_t u = 0;
     int32_t last = -1;
     for (;;) {
         int32_t m = INT32_MAX;
-        for (int l = 0; l < b_h; ++l)
+#pragma unroll kUnroll
+        for (int l = 0; l < bh; ++l)
             if (pos[l] < end[l]) m = min(m, col[pos[l]] / b_w);
         if (m == INT32_MAX) break;
-        for (int l = 0; l < b_h; ++l)
+#pragma unroll kUnroll
+        for (int l = 0; l < bh; ++l)
             while (pos[l] < end[l] && col[pos[l]] / b_w == m) ++pos[l];
         if (m != last) {
             if (out) out[u] = m;
@@ -338,13 +347,14 @@ __device__ int64_t merge_block_row(const int32_t *__restrict__ row_ptr,
     return u;
 }
 
+template <int BH>
 __global__ void bcrs_count(int64_t nbr, int b_h, int b_w,
                            const int32_t *__restrict__ row_ptr,
                            const int32_t *__restrict__ col,
                            int64_t *__restrict__ cnt, int64_t *__restrict__ bad) {
     const int64_t br = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (br >= nbr) return;
-    const int64_t u = merge_block_row(row_ptr, col, br, b_h, b_w, nullptr);
+    const int64_t u = merge_block_row<BH>(row_ptr, col, br, b_h, b_w, nullptr);
     if (u < 0) {
         atomicAdd(reinterpret_cast<unsigned long long *>(bad), 1ull);
         cnt[br] = 0;
@@ -353,6 +363,7 @@ __global__ void bcrs_count(int64_t nbr, int b_h, int b_w,
     }
 }
 
+template <int BH>
 __global__ void bcrs_fill_cols(int64_t nbr, int b_h, int b_w,
                                const int32_t *__restrict__ row_ptr,
                                const int32_t *__restrict__ col,
@@ -364,7 +375,7 @@ __global__ void bcrs_fill_cols(int64_t nbr, int b_h, int b_w,
     const int64_t start = br == 0 ? 0 : incl[br - 1];
     brow_ptr[br + 1] = static_cast<int32_t>(incl[br]);
     if (br == 0) brow_ptr[0] = 0;
-    merge_block_row(row_ptr, col, br, b_h, b_w, bcol + start);
+    merge_block_row<BH>(row_ptr, col, br, b_h, b_w, bcol + start);
 }
 
 template <typename V>
@@ -540,9 +551,17 @@ int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
     if (!rc) rc = smax_hip_zero(ctx, bad, 8);
     int64_t n_blocks = 0, nbad = 0;
     if (!rc) {
-        hipLaunchKernelGGL(bcrs_count, grid_for(nbr, 128), dim3(128), 0,
-                           ctx->stream, nbr, static_cast<int>(b_h),
-                           static_cast<int>(b_w), row_ptr, col, cnt, bad);
+#define SMAX_BCRS_COUNT(BH)                                                              \
+    hipLaunchKernelGGL(bcrs_count<BH>, grid_for(nbr, 128), dim3(128), 0, ctx->stream, nbr, \
+                       static_cast<int>(b_h), static_cast<int>(b_w), row_ptr, col, cnt, bad)
+        switch (b_h) {
+        case 1: SMAX_BCRS_COUNT(1); break;
+        case 2: SMAX_BCRS_COUNT(2); break;
+        case 4: SMAX_BCRS_COUNT(4); break;
+        case 8: SMAX_BCRS_COUNT(8); break;
+        default: SMAX_BCRS_COUNT(0); break;
+        }
+#undef SMAX_BCRS_COUNT
         rc = check_launch(ctx, "bcrs_count");
     }
     if (!rc) rc = scan_inclusive<int64_t, false>(ctx, cnt, incl, nbr, tmp);
@@ -565,10 +584,18 @@ int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
                        "bcrs val");
     if (!rc) rc = smax_hip_zero(ctx, info.val, val_bytes * n_blocks * h_pad * w_pad);
     if (!rc) {
-        hipLaunchKernelGGL(bcrs_fill_cols, grid_for(nbr, 128), dim3(128), 0,
-                           ctx->stream, nbr, static_cast<int>(b_h),
-                           static_cast<int>(b_w), row_ptr, col, incl,
-                           info.row_ptr, info.col);
+#define SMAX_BCRS_FILL(BH)                                                                \
+    hipLaunchKernelGGL(bcrs_fill_cols<BH>, grid_for(nbr, 128), dim3(128), 0, ctx->stream,   \
+                       nbr, static_cast<int>(b_h), static_cast<int>(b_w), row_ptr, col, incl, \
+                       info.row_ptr, info.col)
+        switch (b_h) {
+        case 1: SMAX_BCRS_FILL(1); break;
+        case 2: SMAX_BCRS_FILL(2); break;
+        case 4: SMAX_BCRS_FILL(4); break;
+        case 8: SMAX_BCRS_FILL(8); break;
+        default: SMAX_BCRS_FILL(0); break;
+        }
+#undef SMAX_BCRS_FILL
         rc = check_launch(ctx, "bcrs_fill_cols");
     }
     if (!rc && nnz > 0) {

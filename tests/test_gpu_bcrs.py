@@ -130,6 +130,33 @@ def test_bcrs_scalar_equivalence(ctx):
     b.destroy()
 
 
+@pytest.mark.parametrize("bh,bw", [(3, 3), (8, 2), (16, 4), (6, 1)])
+def test_bcrs_conversion_other_block_heights(ctx, bh, bw):
+    """Block heights outside the register-specialised {1, 2, 4} (8 included)
+    take the generic block-row merge: arrays equal the oracle's conversion
+    and the SpMV equals the oracle's BCRS SpMV."""
+    A = matgen.lomax_band(4800, half=300)  # ascending columns; 4800 = 3*16*100
+    b = gpu_bcrs(ctx, A.n_rows, A.n_cols, A.row_ptr, A.col, A.val, bh, bw, bh, bw, False)
+    want = oracle.crs_to_bcrs(A.n_rows, A.n_cols, A.row_ptr, A.col, A.val, bh, bw, bh, bw,
+                              False)
+    got = b.arrays(np.float64)
+    assert b.info.n_blocks == want["n_blocks"]
+    for f in ("col", "row_ptr"):
+        assert_bitwise(got[f], want[f], f)
+    assert_bitwise(got["val"], want["val"], "val")
+    i = b.info
+    x = matgen.uniform_f64(9, A.n_cols)
+    dx, dy = ctx.upload(x), ctx.zeros(A.n_rows, np.float64)
+    ctx.bcrs_spmv(capi.F64, i.n_block_rows, bh, bw, bh, bw, i.col, i.row_ptr, i.val, False,
+                  dx, dy)
+    assert_bitwise(ctx.download(dy), oracle.bcrs_spmv(i.n_block_rows, bh, bw, bh, bw,
+                                                      want["col"], want["row_ptr"],
+                                                      want["val"], False, x))
+    for d in (dx, dy):
+        d.free()
+    b.destroy()
+
+
 def test_bcrs_conversion_rejects_what_the_reference_rejects(ctx):
     A = matgen.poisson2d(6)  # 36 x 36
     for bh, bw, hp, wp in ((5, 4, 5, 4), (4, 5, 4, 5), (4, 4, 3, 4), (4, 4, 4, 3)):
