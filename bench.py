@@ -468,6 +468,15 @@ def c5_section(ctx, steps, warmup) -> dict:
     T = B.transpose()
     nb = B.info.n_blocks
     out = {"n_rows": A.n_rows, "n_blocks": nb, "values": nb * 16}
+    # the BCRS SpMV (one right-hand side) on the same matrix
+    ba1 = nb * (16 * 2 + 4) + (B.info.n_block_rows + 1) * 4 + 2 * A.n_rows * 4
+    x1 = ctx.upload(matgen.uniform_f32(matgen.SEED_X, A.n_rows))
+    y1 = ctx.zeros(A.n_rows, np.float32)
+    ms = time_launches(ctx, lambda: B.spmv(capi.BF16, x1, y1), steps, warmup)
+    out["spmv"] = {"ms": round(ms, 4), "GBs_alg": round(ba1 / ms / 1e6, 1),
+                   "GFLOPs": round(2 * nb * 16 / ms / 1e6, 1), "B_alg": ba1}
+    x1.free()
+    y1.free()
     for k in (8, 64):
         ba = nb * (16 * 2 + 4) + (B.info.n_block_rows + 1) * 4 + 2 * A.n_rows * k * 4
         X = ctx.upload(matgen.uniform_f32(matgen.SEED_X, A.n_rows * k))
