@@ -145,22 +145,14 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_kernel(
     Y[(r * h_pad + ix) * k + v] = acc;
 }
 
-// Register-blocked SpMM for compile-time block shapes: thread (r, l) keeps
-// the BH outputs of block row r in columns v = l*Q .. l*Q+Q-1, so each X row
-// of a block is loaded once for all BH rows (instead of once per row), and
-// the BH*Q chains give independent fmas.  Q > 1 (16-byte runs: k a multiple
-// of Q, X and Y 16-byte aligned) reads each lane's part of an X row with one
-// 16-byte load.  Per output the fma order is still the reference's (block j,
-// then jx); the next block's values and X entries are loaded while this
-// block's fmas run.
 // One block's values, widened to the accumulation type.  DENSE: h_pad == BH
 // and w_pad == BW, so the block is BH*BW contiguous values (row- or
 // column-major), read as 16-byte loads when that size is a multiple of 16
 // bytes (val is then 16-byte aligned, checked by the host); BF16 blocks are
 // read as 32-bit words holding two values each.
-template <int VT, int BH, int BW, bool DENSE>
+template <int VT, int BH, int BW, bool DENSE, bool CM>
 __device__ __forceinline__ void load_block(const typename vt<VT>::val_t *__restrict__ blk,
-                                           int64_t vs_i, int64_t vs_j, bool col_major,
+                                           int64_t vs_i, int64_t vs_j,
                                            typename vt<VT>::vec_t (&w)[BH][BW]) {
     using T = vt<VT>;
     using A = typename T::vec_t;
@@ -184,7 +176,7 @@ __device__ __forceinline__ void load_block(const typename vt<VT>::val_t *__restr
         for (int ix = 0; ix < BH; ++ix)
 #pragma unroll
             for (int jx = 0; jx < BW; ++jx)
-                w[ix][jx] = col_major ? flat[jx * BH + ix] : flat[ix * BW + jx];
+                w[ix][jx] = CM ? flat[jx * BH + ix] : flat[ix * BW + jx];
     } else {
 #pragma unroll
         for (int jx = 0; jx < BW; ++jx)
@@ -193,7 +185,16 @@ __device__ __forceinline__ void load_block(const typename vt<VT>::val_t *__restr
     }
 }
 
-template <int VT, int BH, int BW, int Q, bool DENSE>
+// Register-blocked SpMM for compile-time block shapes: thread (r, l) keeps
+// the BH outputs of block row r in columns v = l*Q .. l*Q+Q-1, so each X row
+// of a block is loaded once for all BH rows (instead of once per row), and
+// the BH*Q chains give independent fmas.  Q > 1 (16-byte runs: k a multiple
+// of Q, X and Y 16-byte aligned) reads each lane's part of an X row with one
+// 16-byte load.  Blocks go in groups of G: the G column indices, then all G
+// blocks' values and X entries are requested before any fma, so a row of
+// G blocks costs two memory round trips.  Per output the fma order is still
+// the reference's (block j, then jx).
+template <int VT, int BH, int BW, int Q, bool DENSE, bool CM, int G>
 __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     int64_t n_block_rows, int h_pad, int w_pad, int col_major, int64_t k,
     const int32_t *__restrict__ col, const int32_t *__restrict__ row_ptr,
@@ -213,46 +214,33 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     for (int ix = 0; ix < BH; ++ix)
 #pragma unroll
         for (int q = 0; q < Q; ++q) acc[ix][q] = 0;
-    int64_t j = row_ptr[r];
     const int64_t je = row_ptr[r + 1];
-    if (j < je) {
-        A w0[BH][BW];
-        A x0[BW][Q];
-        {
-            const A *xr = X + static_cast<int64_t>(col[j]) * w_pad * k + v;
+    for (int64_t j = row_ptr[r]; j < je; j += G) {
+        int32_t c[G];
 #pragma unroll
-            for (int jx = 0; jx < BW; ++jx) load_run<A, Q>(xr + jx * k, x0[jx]);
-            load_block<VT, BH, BW, DENSE>(val + j * bs, vs_i, vs_j, col_major, w0);
-        }
-        for (; j + 1 < je; ++j) {
-            A w1[BH][BW];
-            A x1[BW][Q];
-            const A *xr = X + static_cast<int64_t>(col[j + 1]) * w_pad * k + v;
+        for (int g = 0; g < G; ++g)
+            if (j + g < je) c[g] = col[j + g];
+        A w[G][BH][BW];
+        A xv[G][BW][Q];
 #pragma unroll
-            for (int jx = 0; jx < BW; ++jx) load_run<A, Q>(xr + jx * k, x1[jx]);
-            load_block<VT, BH, BW, DENSE>(val + (j + 1) * bs, vs_i, vs_j, col_major, w1);
+        for (int g = 0; g < G; ++g)
+            if (j + g < je) {
+                const A *xr = X + static_cast<int64_t>(c[g]) * w_pad * k + v;
 #pragma unroll
-            for (int ix = 0; ix < BH; ++ix)
-#pragma unroll
-                for (int jx = 0; jx < BW; ++jx)
-#pragma unroll
-                    for (int q = 0; q < Q; ++q)
-                        acc[ix][q] = T::fma_(w0[ix][jx], x0[jx][q], acc[ix][q]);
-#pragma unroll
-            for (int jx = 0; jx < BW; ++jx) {
-#pragma unroll
-                for (int q = 0; q < Q; ++q) x0[jx][q] = x1[jx][q];
-#pragma unroll
-                for (int ix = 0; ix < BH; ++ix) w0[ix][jx] = w1[ix][jx];
+                for (int jx = 0; jx < BW; ++jx) load_run<A, Q>(xr + jx * k, xv[g][jx]);
+                load_block<VT, BH, BW, DENSE, CM>(val + (j + g) * bs, vs_i, vs_j, w[g]);
             }
-        }
 #pragma unroll
-        for (int ix = 0; ix < BH; ++ix)
+        for (int g = 0; g < G; ++g)
+            if (j + g < je) {
 #pragma unroll
-            for (int jx = 0; jx < BW; ++jx)
+                for (int ix = 0; ix < BH; ++ix)
 #pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    acc[ix][q] = T::fma_(w0[ix][jx], x0[jx][q], acc[ix][q]);
+                    for (int jx = 0; jx < BW; ++jx)
+#pragma unroll
+                        for (int q = 0; q < Q; ++q)
+                            acc[ix][q] = T::fma_(w[g][ix][jx], xv[g][jx][q], acc[ix][q]);
+            }
     }
 #pragma unroll
     for (int ix = 0; ix < BH; ++ix) store_run<A, Q>(Y + (r * h_pad + ix) * k + v, acc[ix]);
@@ -277,14 +265,24 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
     constexpr bool kBlock16 = (BH * BW * sizeof(typename T::val_t)) % 16 == 0;
     const bool dense = h_pad == BH && w_pad == BW && (!kBlock16 || aligned16(val, val));
     const dim3 g = grid_for(n_block_rows * (runs ? k / Q : k), kBThreads);
-#define SMAX_BCRS_REG_LAUNCH(QQ, DD)                                                        \
-    hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW, QQ, DD>), g, dim3(kBThreads), 0,          \
+    // block groups: 8 when a single thread holds the whole block row (k = 1),
+    // else 2 (measured on the 96^3 block matrix: k = 1 0.108 -> 0.058 ms,
+    // k = 8 0.173 -> 0.160 ms, k = 64 0.663 -> 0.649 ms in BF16)
+    const bool single = (runs ? k / Q : k) == 1;
+#define SMAX_BCRS_REG_LAUNCH(QQ, DD, CC, GG)                                                \
+    hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW, QQ, DD, CC, GG>), g, dim3(kBThreads), 0,  \
                        ctx->stream, n_block_rows, static_cast<int>(h_pad),                 \
                        static_cast<int>(w_pad), cm ? 1 : 0, k, col, row_ptr, v, xv, yv)
-    if (runs && dense) SMAX_BCRS_REG_LAUNCH(Q, true);
-    else if (runs) SMAX_BCRS_REG_LAUNCH(Q, false);
-    else if (dense) SMAX_BCRS_REG_LAUNCH(1, true);
-    else SMAX_BCRS_REG_LAUNCH(1, false);
+#define SMAX_BCRS_REG_G(QQ, DD, CC)                                                         \
+    if (single) SMAX_BCRS_REG_LAUNCH(QQ, DD, CC, 8);                                        \
+    else SMAX_BCRS_REG_LAUNCH(QQ, DD, CC, 2)
+    if (runs && dense && cm) { SMAX_BCRS_REG_G(Q, true, true); }
+    else if (runs && dense) { SMAX_BCRS_REG_G(Q, true, false); }
+    else if (runs) { SMAX_BCRS_REG_G(Q, false, false); }
+    else if (dense && cm) { SMAX_BCRS_REG_G(1, true, true); }
+    else if (dense) { SMAX_BCRS_REG_G(1, true, false); }
+    else { SMAX_BCRS_REG_G(1, false, false); }
+#undef SMAX_BCRS_REG_G
 #undef SMAX_BCRS_REG_LAUNCH
 }
 
