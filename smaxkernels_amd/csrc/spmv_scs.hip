@@ -336,28 +336,31 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmm_vec(
 
 template <int VT, int K>
 static void launch_scs_spmm_vec(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_t n_rows,
-                                int64_t cpb, int64_t bpc, int64_t blocks,
+                                int64_t cpb, int64_t bpc, int64_t blocks, int64_t lds_bytes,
                                 const int32_t *chunk_ptr, const int32_t *chunk_len,
                                 const int32_t *slot_len, const int32_t *order,
                                 const int32_t *col, const typename vt<VT>::val_t *val,
                                 const int32_t *slot_row, const typename vt<VT>::vec_t *X,
                                 typename vt<VT>::vec_t *Y) {
+    const unsigned lds = static_cast<unsigned>(lds_bytes);
     if (slot_len)
         hipLaunchKernelGGL((scs_spmm_vec<VT, true, K>), dim3(static_cast<unsigned>(blocks)),
-                           dim3(kScsThreads), 0, ctx->stream, C, n_chunks, n_rows, cpb, bpc,
+                           dim3(kScsThreads), lds, ctx->stream, C, n_chunks, n_rows, cpb, bpc,
                            chunk_ptr, chunk_len, slot_len, order, col, val, slot_row, X, Y);
     else
         hipLaunchKernelGGL((scs_spmm_vec<VT, false, K>), dim3(static_cast<unsigned>(blocks)),
-                           dim3(kScsThreads), 0, ctx->stream, C, n_chunks, n_rows, cpb, bpc,
+                           dim3(kScsThreads), lds, ctx->stream, C, n_chunks, n_rows, cpb, bpc,
                            chunk_ptr, chunk_len, slot_len, order, col, val, slot_row, X, Y);
 }
 
+// lds_bytes: dynamic LDS reserved per workgroup of the 16-byte-run kernel
+// (an occupancy cap, 0 = none).
 template <int VT>
 static int launch_scs_spmm(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_t n_rows,
                            int64_t k, const int32_t *chunk_ptr, const int32_t *chunk_len,
                            const int32_t *slot_len, const int32_t *order,
                            const int32_t *col, const void *val, const int32_t *slot_row,
-                           const void *X, void *Y) {
+                           const void *X, void *Y, int64_t lds_bytes = 0) {
     using T = vt<VT>;
     using A = typename T::vec_t;
     constexpr int64_t Q = 16 / sizeof(A);
@@ -377,8 +380,9 @@ static int launch_scs_spmm(smax_hip_ctx ctx, int64_t C, int64_t n_chunks, int64_
 #define SMAX_SCS_SPMM_VEC(KK)                                                                \
     case KK:                                                                                  \
         if constexpr (KK >= Q)                                                                \
-            launch_scs_spmm_vec<VT, KK>(ctx, C, n_chunks, n_rows, cpb, bpc, blocks, chunk_ptr, \
-                                        chunk_len, slot_len, order, col, v, slot_row, xv, yv); \
+            launch_scs_spmm_vec<VT, KK>(ctx, C, n_chunks, n_rows, cpb, bpc, blocks, lds_bytes, \
+                                        chunk_ptr, chunk_len, slot_len, order, col, v,         \
+                                        slot_row, xv, yv);                                     \
         break;
             SMAX_SCS_SPMM_VEC(2)
             SMAX_SCS_SPMM_VEC(4)
@@ -765,11 +769,15 @@ int smax_hip_scs_apply_spmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
         return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
                     "smax_hip_scs_apply_spmm: value type %d does not match %lld-byte "
                     "values", value_type, (long long)in.val_bytes);
+    // the SpMV's occupancy cap pays off for the SpMM too when long chunks lead
+    // (0.40 -> 0.37 ms on the 1M-row band matrix at sigma = 65536, k = 32);
+    // without long chunks it costs (0.27 -> 0.28 ms on its transpose)
+    const int64_t lds = A->sched.order_lf ? A->sched.lds_cap_bytes : 0;
     SMAX_HIP_DISPATCH_VT(value_type,
                          return launch_scs_spmm<VT>(ctx, in.C, in.n_chunks, in.n_rows, k,
                                                     in.chunk_ptr, in.chunk_lengths,
                                                     in.slot_len, A->sched.order_lf, in.col,
-                                                    in.val, in.slot_row, X, Y));
+                                                    in.val, in.slot_row, X, Y, lds));
     return SMAX_HIP_ERROR_UNSUPPORTED;
 }
 
