@@ -27,6 +27,7 @@ struct smax_hip_crs_plan_s {
     int64_t n_rows = 0, nnz = 0, n_blocks = 0;
     int32_t *block_start = nullptr;  // n_blocks + 1 row indices
     int32_t *row_order = nullptr;    // row-major SpMM execution order (n_rows)
+    int64_t spmm_lds = 0;            // dynamic LDS per SpMM workgroup (occupancy cap)
 };
 
 namespace smaxhip {
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_rowvec(
 // their row strides are 16-byte aligned; false otherwise.
 template <int VT>
 static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
-                                   const int32_t *order, const int32_t *row_ptr,
+                                   const int32_t *order, unsigned lds, const int32_t *row_ptr,
                                    const int32_t *col,
                                    const void *val, const void *X, int64_t ldx, void *Y,
                                    int64_t ldy) {
@@ -297,8 +298,8 @@ static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
 #define SMAX_CRS_ROWVEC(KK)                                                                  \
     case KK:                                                                                  \
         if constexpr (KK >= Q)                                                                \
-            hipLaunchKernelGGL((crs_spmm_rowvec<VT, KK>), g, dim3(kCrsThreads), 0, ctx->stream, \
-                               n_rows, order, row_ptr, col, v, xv, ldx, yv, ldy);             \
+            hipLaunchKernelGGL((crs_spmm_rowvec<VT, KK>), g, dim3(kCrsThreads), lds,            \
+                               ctx->stream, n_rows, order, row_ptr, col, v, xv, ldx, yv, ldy); \
         return true;
     switch (k) {
         SMAX_CRS_ROWVEC(2)
@@ -321,10 +322,12 @@ static bool launch_crs_spmm_rowvec(smax_hip_ctx ctx, int64_t n_rows, int64_t k,
 // the rows sharing a wave have similar lengths.  Results do not depend on
 // the order.
 constexpr int64_t kSpmmWindow = 65536;
+constexpr int64_t kSpmmCapLds = 53 * 1024;
 
 static int build_row_order(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
-                           const int32_t *row_ptr, int32_t **out) {
+                           const int32_t *row_ptr, int32_t **out, int64_t *n_long) {
     *out = nullptr;
+    *n_long = 0;
     std::vector<int32_t> rp(static_cast<size_t>(n_rows + 1));
     SMAX_HIP_CALL(ctx, hipMemcpyAsync(rp.data(), row_ptr, sizeof(int32_t) * (n_rows + 1),
                                       hipMemcpyDeviceToHost, ctx->stream));
@@ -337,6 +340,7 @@ static int build_row_order(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     ord.reserve(static_cast<size_t>(n_rows));
     for (int64_t r = 0; r < n_rows; ++r)
         if (len(static_cast<int32_t>(r)) >= thr) ord.push_back(static_cast<int32_t>(r));
+    *n_long = static_cast<int64_t>(ord.size());
     auto longer = [&](int32_t a, int32_t b) { return len(a) > len(b); };
     std::stable_sort(ord.begin(), ord.end(), longer);
     for (int64_t w = 0; w < n_rows; w += kSpmmWindow) {
@@ -414,7 +418,8 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     dev_free(flag);
     dev_free(tmp);
     int32_t *order = nullptr;
-    if (!rc) rc = build_row_order(ctx, n_rows, nnz, row_ptr, &order);
+    int64_t n_long = 0;
+    if (!rc) rc = build_row_order(ctx, n_rows, nnz, row_ptr, &order, &n_long);
     if (rc) {
         dev_free(starts);
         dev_free(order);
@@ -426,6 +431,10 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     p->n_blocks = n_blocks;
     p->block_start = starts;
     p->row_order = order;
+    // With long rows leading, three workgroups per CU (53 KB of LDS each out
+    // of 160 KB) run the row-major SpMM faster than full occupancy: 0.46 ->
+    // 0.40 ms on the 1M-row band matrix, k = 32.
+    p->spmm_lds = n_long > 0 ? kSpmmCapLds : 0;
     *out = p;
     return SMAX_HIP_SUCCESS;
 }
@@ -503,14 +512,16 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
         SMAX_HIP_REQUIRE(ctx, plan->n_rows == n_rows && plan->nnz == nnz,
                          "smax_hip_crs_spmm: plan built for another pattern");
     const int32_t *order = plan ? plan->row_order : nullptr;
+    const int64_t spmm_lds = plan ? plan->spmm_lds : 0;
     const int64_t threads = n_rows * k;
     SMAX_HIP_DISPATCH_VT(value_type, {
         using T = vt<VT>;
         auto v = static_cast<const typename T::val_t *>(val);
         auto xv = static_cast<const typename T::vec_t *>(X);
         auto yv = static_cast<typename T::vec_t *>(Y);
-        if (row_major && launch_crs_spmm_rowvec<VT>(ctx, n_rows, k, order, row_ptr, col, val,
-                                                    X, ldx, Y, ldy)) {
+        if (row_major && launch_crs_spmm_rowvec<VT>(ctx, n_rows, k, order,
+                                                    static_cast<unsigned>(spmm_lds), row_ptr,
+                                                    col, val, X, ldx, Y, ldy)) {
             SMAX_HIP_LAUNCHED(ctx, "crs_spmm_rowvec");
             return SMAX_HIP_SUCCESS;
         }
