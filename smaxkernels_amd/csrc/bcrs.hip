@@ -303,56 +303,58 @@ static void launch_bcrs_spmm(smax_hip_ctx ctx, int64_t threads, int64_t n_block_
 }
 
 // ------------------------------------------------------------ conversion
-// Sorted unique block columns of block row br: a merge of its b_h rows, whose
-// block-column sequences must be non-decreasing (the reference's fill loop,
-// convert_formats.hpp:222-234, throws otherwise).  Writes them to out (if
-// non-null) and returns their count, or -1 on an out-of-order row.
+// Sorted unique block columns of each block row: a merge of its b_h rows,
+// whose block-column sequences must be non-decreasing (the reference's fill
+// loop, convert_formats.hpp:222-234, throws otherwise).
 //
-// BH > 0 fixes b_h at compile time, so the b_h cursors live in registers;
-// BH == 0 takes b_h (<= 64) at run time.
-template <int BH>
+// A group of GS lanes (GS the power of two >= b_h) takes one block row, lane
+// l its CRS row br*b_h + l, reading that row front to back (neighbouring
+// lanes read neighbouring rows).  Each step the group's smallest head block
+// column m is found with an xor-butterfly; every lane then steps past its
+// entries in column block m, and m is the block row's next block column.
+// Returns (on every lane of the group) the number of block columns, or -1
+// when one of the rows goes backwards; with out, lane 0 writes them.
+template <int GS>
 __device__ int64_t merge_block_row(const int32_t *__restrict__ row_ptr,
-                                   const int32_t *__restrict__ col, int64_t br,
+                                   const int32_t *__restrict__ col, int64_t br, bool valid,
                                    int b_h, int b_w, int32_t *out) {
-    constexpr int kMaxBh = BH > 0 ? BH : 64;
-    constexpr int kUnroll = BH > 0 ? BH : 1;
-    const int bh = BH > 0 ? BH : b_h;
-    int32_t pos[kMaxBh], end[kMaxBh];
-#pragma unroll kUnroll
-    for (int l = 0; l < bh; ++l) {
-        pos[l] = row_ptr[br * bh + l];
-        end[l] = row_ptr[br * bh + l + 1];
-        for (int32_t j = pos[l] + 1; j < end[l]; ++j)
-            if (col[j] / b_w < col[j - 1] / b_w) return -1;
+    const int lane = static_cast<int>(threadIdx.x) & (GS - 1);
+    const bool mine = valid && lane < b_h;
+    int32_t pos = 0, end = 0;
+    int bad = 0;
+    if (mine) {
+        const int64_t r = br * b_h + lane;
+        pos = row_ptr[r];
+        end = row_ptr[r + 1];
+        for (int32_t j = pos + 1; j < end; ++j)
+            bad |= col[j] / b_w < col[j - 1] / b_w;
     }
+#pragma unroll
+    for (int off = GS / 2; off >= 1; off >>= 1) bad |= __shfl_xor(bad, off, GS);
+    if (bad) return -1;
     int64_t u = 0;
-    int32_t last = -1;
     for (;;) {
-        int32_t m = INT32_MAX;
-#pragma unroll kUnroll
-        for (int l = 0; l < bh; ++l)
-            if (pos[l] < end[l]) m = min(m, col[pos[l]] / b_w);
+        int32_t m = pos < end ? col[pos] / b_w : INT32_MAX;
+#pragma unroll
+        for (int off = GS / 2; off >= 1; off >>= 1) m = min(m, __shfl_xor(m, off, GS));
         if (m == INT32_MAX) break;
-#pragma unroll kUnroll
-        for (int l = 0; l < bh; ++l)
-            while (pos[l] < end[l] && col[pos[l]] / b_w == m) ++pos[l];
-        if (m != last) {
-            if (out) out[u] = m;
-            ++u;
-            last = m;
-        }
+        while (pos < end && col[pos] / b_w == m) ++pos;
+        if (out && lane == 0) out[u] = m;
+        ++u;
     }
     return u;
 }
 
-template <int BH>
-__global__ void bcrs_count(int64_t nbr, int b_h, int b_w,
-                           const int32_t *__restrict__ row_ptr,
-                           const int32_t *__restrict__ col,
-                           int64_t *__restrict__ cnt, int64_t *__restrict__ bad) {
-    const int64_t br = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (br >= nbr) return;
-    const int64_t u = merge_block_row<BH>(row_ptr, col, br, b_h, b_w, nullptr);
+template <int GS>
+__global__ __launch_bounds__(256) void bcrs_count(int64_t nbr, int b_h, int b_w,
+                                                  const int32_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ col,
+                                                  int64_t *__restrict__ cnt,
+                                                  int64_t *__restrict__ bad) {
+    const int64_t br = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / GS;
+    const bool valid = br < nbr;
+    const int64_t u = merge_block_row<GS>(row_ptr, col, br, valid, b_h, b_w, nullptr);
+    if (!valid || (threadIdx.x & (GS - 1)) != 0) return;
     if (u < 0) {
         atomicAdd(reinterpret_cast<unsigned long long *>(bad), 1ull);
         cnt[br] = 0;
@@ -361,19 +363,28 @@ __global__ void bcrs_count(int64_t nbr, int b_h, int b_w,
     }
 }
 
-template <int BH>
-__global__ void bcrs_fill_cols(int64_t nbr, int b_h, int b_w,
-                               const int32_t *__restrict__ row_ptr,
-                               const int32_t *__restrict__ col,
-                               const int64_t *__restrict__ incl,
-                               int32_t *__restrict__ brow_ptr,
-                               int32_t *__restrict__ bcol) {
-    const int64_t br = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (br >= nbr) return;
-    const int64_t start = br == 0 ? 0 : incl[br - 1];
-    brow_ptr[br + 1] = static_cast<int32_t>(incl[br]);
-    if (br == 0) brow_ptr[0] = 0;
-    merge_block_row<BH>(row_ptr, col, br, b_h, b_w, bcol + start);
+template <int GS>
+__global__ __launch_bounds__(256) void bcrs_fill_cols(int64_t nbr, int b_h, int b_w,
+                                                      const int32_t *__restrict__ row_ptr,
+                                                      const int32_t *__restrict__ col,
+                                                      const int64_t *__restrict__ incl,
+                                                      int32_t *__restrict__ brow_ptr,
+                                                      int32_t *__restrict__ bcol) {
+    const int64_t br = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / GS;
+    const bool valid = br < nbr;
+    const int64_t start = valid ? (br == 0 ? 0 : incl[br - 1]) : 0;
+    if (valid && (threadIdx.x & (GS - 1)) == 0) {
+        brow_ptr[br + 1] = static_cast<int32_t>(incl[br]);
+        if (br == 0) brow_ptr[0] = 0;
+    }
+    merge_block_row<GS>(row_ptr, col, br, valid, b_h, b_w, bcol + start);
+}
+
+// Lanes per block row of the conversion: the power of two >= b_h (<= 64).
+inline int bcrs_group(int64_t b_h) {
+    int g = 1;
+    while (g < b_h) g <<= 1;
+    return g;
 }
 
 template <typename V>
@@ -555,15 +566,17 @@ int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
     if (!rc) rc = smax_hip_zero(ctx, bad, 8);
     int64_t n_blocks = 0, nbad = 0;
     if (!rc) {
-#define SMAX_BCRS_COUNT(BH)                                                              \
-    hipLaunchKernelGGL(bcrs_count<BH>, grid_for(nbr, 128), dim3(128), 0, ctx->stream, nbr, \
-                       static_cast<int>(b_h), static_cast<int>(b_w), row_ptr, col, cnt, bad)
-        switch (b_h) {
+#define SMAX_BCRS_COUNT(GS)                                                              \
+    hipLaunchKernelGGL(bcrs_count<GS>, grid_for(nbr * GS, 256), dim3(256), 0, ctx->stream, \
+                       nbr, static_cast<int>(b_h), static_cast<int>(b_w), row_ptr, col, cnt, bad)
+        switch (bcrs_group(b_h)) {
         case 1: SMAX_BCRS_COUNT(1); break;
         case 2: SMAX_BCRS_COUNT(2); break;
         case 4: SMAX_BCRS_COUNT(4); break;
         case 8: SMAX_BCRS_COUNT(8); break;
-        default: SMAX_BCRS_COUNT(0); break;
+        case 16: SMAX_BCRS_COUNT(16); break;
+        case 32: SMAX_BCRS_COUNT(32); break;
+        default: SMAX_BCRS_COUNT(64); break;
         }
 #undef SMAX_BCRS_COUNT
         rc = check_launch(ctx, "bcrs_count");
@@ -588,16 +601,18 @@ int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
                        "bcrs val");
     if (!rc) rc = smax_hip_zero(ctx, info.val, val_bytes * n_blocks * h_pad * w_pad);
     if (!rc) {
-#define SMAX_BCRS_FILL(BH)                                                                \
-    hipLaunchKernelGGL(bcrs_fill_cols<BH>, grid_for(nbr, 128), dim3(128), 0, ctx->stream,   \
-                       nbr, static_cast<int>(b_h), static_cast<int>(b_w), row_ptr, col, incl, \
+#define SMAX_BCRS_FILL(GS)                                                                 \
+    hipLaunchKernelGGL(bcrs_fill_cols<GS>, grid_for(nbr * GS, 256), dim3(256), 0, ctx->stream, \
+                       nbr, static_cast<int>(b_h), static_cast<int>(b_w), row_ptr, col, incl,  \
                        info.row_ptr, info.col)
-        switch (b_h) {
+        switch (bcrs_group(b_h)) {
         case 1: SMAX_BCRS_FILL(1); break;
         case 2: SMAX_BCRS_FILL(2); break;
         case 4: SMAX_BCRS_FILL(4); break;
         case 8: SMAX_BCRS_FILL(8); break;
-        default: SMAX_BCRS_FILL(0); break;
+        case 16: SMAX_BCRS_FILL(16); break;
+        case 32: SMAX_BCRS_FILL(32); break;
+        default: SMAX_BCRS_FILL(64); break;
         }
 #undef SMAX_BCRS_FILL
         rc = check_launch(ctx, "bcrs_fill_cols");
