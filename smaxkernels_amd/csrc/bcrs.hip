@@ -387,32 +387,28 @@ inline int bcrs_group(int64_t b_h) {
     return g;
 }
 
+// Places every CRS value in its block: one thread per CRS row walks the row
+// and its block row's (sorted) block columns together; the row's block
+// columns are non-decreasing (checked by bcrs_count), so each entry's block
+// is found by stepping forward.
 template <typename V>
-__global__ void bcrs_fill_vals(int64_t nnz, int64_t n_rows, int b_h, int b_w,
-                               int h_pad, int w_pad, int col_major,
-                               const int32_t *__restrict__ row_ptr,
-                               const int32_t *__restrict__ col,
-                               const V *__restrict__ val,
+__global__ void bcrs_fill_vals(int64_t n_rows, int b_h, int b_w, int h_pad, int w_pad,
+                               int col_major, const int32_t *__restrict__ row_ptr,
+                               const int32_t *__restrict__ col, const V *__restrict__ val,
                                const int32_t *__restrict__ brow_ptr,
-                               const int32_t *__restrict__ bcol,
-                               V *__restrict__ bval) {
-    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= nnz) return;
-    int64_t lo = 0, hi = n_rows;  // row of j: row_ptr[lo] <= j < row_ptr[hi]
-    while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (row_ptr[mid] <= j) lo = mid; else hi = mid;
+                               const int32_t *__restrict__ bcol, V *__restrict__ bval) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    const int64_t br = r / b_h, lr = r - br * b_h;
+    int64_t a = brow_ptr[br];
+    const int64_t bs = static_cast<int64_t>(h_pad) * w_pad;
+    for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; ++j) {
+        const int32_t c = col[j], bc = c / b_w;
+        while (bcol[a] < bc) ++a;
+        const int64_t inc = col_major ? static_cast<int64_t>(h_pad) * (c % b_w) + lr
+                                      : static_cast<int64_t>(w_pad) * lr + (c % b_w);
+        bval[a * bs + inc] = val[j];
     }
-    const int64_t r = lo, br = r / b_h, lr = r - br * b_h;
-    const int32_t c = col[j], bc = c / b_w;
-    int64_t a = brow_ptr[br], b = brow_ptr[br + 1];  // lower_bound of bc
-    while (a < b) {
-        const int64_t m = (a + b) >> 1;
-        if (bcol[m] < bc) a = m + 1; else b = m;
-    }
-    const int64_t inc = col_major ? static_cast<int64_t>(h_pad) * (c % b_w) + lr
-                                  : static_cast<int64_t>(w_pad) * lr + (c % b_w);
-    bval[a * h_pad * w_pad + inc] = val[j];
 }
 
 static int check_bcrs(smax_hip_ctx ctx, const char *fn, int value_type,
@@ -618,25 +614,24 @@ int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
         rc = check_launch(ctx, "bcrs_fill_cols");
     }
     if (!rc && nnz > 0) {
-        const dim3 g = grid_for(nnz, 256);
         const int bh = static_cast<int>(b_h), bw = static_cast<int>(b_w);
         const int hp = static_cast<int>(h_pad), wp = static_cast<int>(w_pad);
         const int cm = block_col_major ? 1 : 0;
         if (val_bytes == 8)
-            hipLaunchKernelGGL(bcrs_fill_vals<uint64_t>, g, dim3(256), 0,
-                               ctx->stream, nnz, n_rows, bh, bw, hp, wp, cm,
+            hipLaunchKernelGGL(bcrs_fill_vals<uint64_t>, grid_for(n_rows, 256), dim3(256), 0,
+                               ctx->stream, n_rows, bh, bw, hp, wp, cm,
                                row_ptr, col, static_cast<const uint64_t *>(val),
                                info.row_ptr, info.col,
                                static_cast<uint64_t *>(info.val));
         else if (val_bytes == 4)
-            hipLaunchKernelGGL(bcrs_fill_vals<uint32_t>, g, dim3(256), 0,
-                               ctx->stream, nnz, n_rows, bh, bw, hp, wp, cm,
+            hipLaunchKernelGGL(bcrs_fill_vals<uint32_t>, grid_for(n_rows, 256), dim3(256), 0,
+                               ctx->stream, n_rows, bh, bw, hp, wp, cm,
                                row_ptr, col, static_cast<const uint32_t *>(val),
                                info.row_ptr, info.col,
                                static_cast<uint32_t *>(info.val));
         else
-            hipLaunchKernelGGL(bcrs_fill_vals<uint16_t>, g, dim3(256), 0,
-                               ctx->stream, nnz, n_rows, bh, bw, hp, wp, cm,
+            hipLaunchKernelGGL(bcrs_fill_vals<uint16_t>, grid_for(n_rows, 256), dim3(256), 0,
+                               ctx->stream, n_rows, bh, bw, hp, wp, cm,
                                row_ptr, col, static_cast<const uint16_t *>(val),
                                info.row_ptr, info.col,
                                static_cast<uint16_t *>(info.val));

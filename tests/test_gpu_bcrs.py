@@ -157,6 +157,21 @@ def test_bcrs_conversion_other_block_heights(ctx, bh, bw):
     b.destroy()
 
 
+def test_bcrs_conversion_duplicate_entries_last_wins(ctx):
+    """A CRS row may list a column twice (Matrix Market files can): the
+    reference places the values in order, so the later one stays; the GPU
+    conversion places each row's values in order too."""
+    rp = np.array([0, 3, 4, 6, 7], np.int32)
+    col = np.array([0, 1, 1, 2, 2, 2, 3], np.int32)
+    val = np.array([1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0])
+    b = gpu_bcrs(ctx, 4, 4, rp, col, val, 2, 2, 2, 2, False)
+    want = oracle.crs_to_bcrs(4, 4, rp, col, val, 2, 2, 2, 2, False)
+    got = b.arrays(np.float64)
+    for f in ("col", "row_ptr", "val"):
+        assert_bitwise(got[f], want[f], f)
+    b.destroy()
+
+
 def test_bcrs_conversion_rejects_what_the_reference_rejects(ctx):
     A = matgen.poisson2d(6)  # 36 x 36
     for bh, bw, hp, wp in ((5, 4, 5, 4), (4, 5, 4, 5), (4, 4, 3, 4), (4, 4, 4, 3)):
