@@ -328,7 +328,9 @@ typedef struct {
 /* CRS -> BCRS on the GPU, identical to SMAX::Utils::convert_crs_to_bcrs
  * (src/utils/convert_formats.hpp:134-247).  Rejects b_h/b_w that do not
  * divide the shape, padding smaller than the block, and rows whose columns
- * are not ascending (the reference throws for each). */
+ * are not ascending (the reference throws for each); also rejects column
+ * indices outside [0, n_cols), which the reference would place outside
+ * their block.  A column listed twice in a row keeps its later value. */
 int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
                            int64_t nnz, const int32_t *row_ptr,
                            const int32_t *col, const void *val,

@@ -14,6 +14,8 @@
 #include "common.hpp"
 #include "scan.hpp"
 
+#include <type_traits>
+
 struct smax_hip_bcrs_s {
     smax_hip_bcrs_info info;
 };
@@ -307,77 +309,159 @@ static void launch_bcrs_spmm(smax_hip_ctx ctx, int64_t threads, int64_t n_block_
 // whose block-column sequences must be non-decreasing (the reference's fill
 // loop, convert_formats.hpp:222-234, throws otherwise).
 //
-// A group of GS lanes (GS the power of two >= b_h) takes one block row, lane
-// l its CRS row br*b_h + l, reading that row front to back (neighbouring
-// lanes read neighbouring rows).  Each step the group's smallest head block
-// column m is found with an xor-butterfly; every lane then steps past its
-// entries in column block m, and m is the block row's next block column.
-// Returns (on every lane of the group) the number of block columns, or -1
-// when one of the rows goes backwards; with out, lane 0 writes them.
-template <int GS>
-__device__ int64_t merge_block_row(const int32_t *__restrict__ row_ptr,
-                                   const int32_t *__restrict__ col, int64_t br, bool valid,
-                                   int b_h, int b_w, int32_t *out) {
-    const int lane = static_cast<int>(threadIdx.x) & (GS - 1);
-    const bool mine = valid && lane < b_h;
-    int32_t pos = 0, end = 0;
-    int bad = 0;
-    if (mine) {
-        const int64_t r = br * b_h + lane;
-        pos = row_ptr[r];
-        end = row_ptr[r + 1];
-        for (int32_t j = pos + 1; j < end; ++j)
-            bad |= col[j] / b_w < col[j - 1] / b_w;
+// A workgroup takes 256/GS consecutive block rows (GS the power of two >=
+// b_h), whose CRS entries are one contiguous range; when the range fits the
+// LDS tile it is loaded once, coalesced (columns, and values in the fill
+// pass), and the merge reads LDS.  A group of GS lanes takes one block row,
+// lane l its CRS row br*b_h + l, read front to back.  Each step the group's
+// smallest head block column m is found with an xor-butterfly; every lane
+// then steps past its entries in block column m, and m is the block row's
+// next block column.  In the fill pass a lane places each value it steps
+// past, in row order, so of two entries with the same column the later one
+// stays, as in the reference's fill loop.
+
+// c / b_w for 0 <= c < 2^31 as a multiply and shift: with l = ceil(log2
+// b_w) and m = ceil(2^(31+l) / b_w), m*b_w - 2^(31+l) < b_w <= 2^l, which
+// makes floor(c*m / 2^(31+l)) exact for every 31-bit c (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994,
+// Theorem 4.2).
+struct BlockDiv {
+    uint32_t m;
+    int shift;
+    __device__ int32_t operator()(int32_t c) const {
+        return static_cast<int32_t>(
+            (static_cast<uint64_t>(static_cast<uint32_t>(c)) * m) >> shift);
     }
+};
+
+inline BlockDiv block_div(int64_t b_w) {
+    int l = 0;
+    while ((int64_t{1} << l) < b_w) ++l;
+    const uint64_t m = ((uint64_t{1} << (31 + l)) + b_w - 1) / b_w;
+    return {static_cast<uint32_t>(m), 31 + l};
+}
+
+template <typename V>
+struct BcrsBuild {
+    int64_t nbr, n_cols;
+    int b_h, b_w, h_pad, w_pad, col_major;
+    int cap;  // entries the LDS tile holds
+    BlockDiv div;
+    const int32_t *row_ptr, *col;
+    const V *val;
+    int64_t *cnt;        // count pass: blocks per block row
+    int64_t *bad;        // count pass: [0] descending rows, [1] columns out of range
+    const int64_t *incl; // fill pass: inclusive scan of cnt
+    int32_t *brow_ptr, *bcol;
+    V *bval;
+};
+
+// Entry j of the CRS arrays, from the LDS tile (which starts at entry
+// base) or from memory.
+template <bool STAGED, typename T>
+struct Entries {
+    const T *p;
+    int32_t base;
+    __device__ T operator[](int32_t j) const { return STAGED ? p[j - base] : p[j]; }
+};
+
+template <int GS, bool FILL, bool STAGED, typename V>
+__device__ void merge_block_row(const BcrsBuild<V> &a, int64_t br, bool row_valid,
+                                Entries<STAGED, int32_t> cs, Entries<STAGED, V> vs) {
+    const int lane = static_cast<int>(threadIdx.x) & (GS - 1);
+    const bool mine = row_valid && lane < a.b_h;
+    int32_t pos = 0, end = 0;
+    if (mine) {
+        const int64_t r = br * a.b_h + lane;
+        pos = a.row_ptr[r];
+        end = a.row_ptr[r + 1];
+    }
+    int desc = 0, range = 0;
+    if (!FILL && mine) {
+        int32_t prev = -1;
+        for (int32_t j = pos; j < end; ++j) {
+            const int32_t c = cs[j];
+            range |= static_cast<uint32_t>(c) >= static_cast<uint64_t>(a.n_cols);
+            const int32_t bc = a.div(c);
+            desc |= bc < prev;
+            prev = bc;
+        }
+    }
+    if (!FILL) {
 #pragma unroll
-    for (int off = GS / 2; off >= 1; off >>= 1) bad |= __shfl_xor(bad, off, GS);
-    if (bad) return -1;
+        for (int off = GS / 2; off >= 1; off >>= 1) {
+            desc |= __shfl_xor(desc, off, GS);
+            range |= __shfl_xor(range, off, GS);
+        }
+        if (desc || range) {
+            if (row_valid && lane == 0) {
+                a.cnt[br] = 0;
+                if (desc) atomicAdd(reinterpret_cast<unsigned long long *>(a.bad), 1ull);
+                if (range) atomicAdd(reinterpret_cast<unsigned long long *>(a.bad + 1), 1ull);
+            }
+            return;
+        }
+    }
+    const int64_t start = FILL && row_valid ? (br == 0 ? 0 : a.incl[br - 1]) : 0;
+    const int64_t bs = static_cast<int64_t>(a.h_pad) * a.w_pad;
     int64_t u = 0;
     for (;;) {
-        int32_t m = pos < end ? col[pos] / b_w : INT32_MAX;
+        int32_t m = pos < end ? a.div(cs[pos]) : INT32_MAX;
 #pragma unroll
         for (int off = GS / 2; off >= 1; off >>= 1) m = min(m, __shfl_xor(m, off, GS));
         if (m == INT32_MAX) break;
-        while (pos < end && col[pos] / b_w == m) ++pos;
-        if (out && lane == 0) out[u] = m;
+        const int64_t blk = start + u;
+        while (pos < end) {
+            const int32_t c = cs[pos];
+            if (a.div(c) != m) break;
+            if (FILL) {
+                const int64_t in = c - static_cast<int64_t>(m) * a.b_w;
+                const int64_t inc = a.col_major ? static_cast<int64_t>(a.h_pad) * in + lane
+                                                : static_cast<int64_t>(a.w_pad) * lane + in;
+                a.bval[blk * bs + inc] = vs[pos];
+            }
+            ++pos;
+        }
+        if (FILL && lane == 0) a.bcol[blk] = m;
         ++u;
     }
-    return u;
+    if (row_valid && lane == 0) {
+        if (FILL) {
+            a.brow_ptr[br + 1] = static_cast<int32_t>(a.incl[br]);
+            if (br == 0) a.brow_ptr[0] = 0;
+        } else {
+            a.cnt[br] = u;
+        }
+    }
 }
 
-template <int GS>
-__global__ __launch_bounds__(256) void bcrs_count(int64_t nbr, int b_h, int b_w,
-                                                  const int32_t *__restrict__ row_ptr,
-                                                  const int32_t *__restrict__ col,
-                                                  int64_t *__restrict__ cnt,
-                                                  int64_t *__restrict__ bad) {
-    const int64_t br = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / GS;
-    const bool valid = br < nbr;
-    const int64_t u = merge_block_row<GS>(row_ptr, col, br, valid, b_h, b_w, nullptr);
-    if (!valid || (threadIdx.x & (GS - 1)) != 0) return;
-    if (u < 0) {
-        atomicAdd(reinterpret_cast<unsigned long long *>(bad), 1ull);
-        cnt[br] = 0;
+// Count pass (FILL false): blocks per block row, descending rows and
+// out-of-range columns.  Fill pass: block row pointers, block columns and
+// values (into zeroed blocks).  Dynamic LDS: cap columns, then (fill pass)
+// cap values.
+template <int GS, bool FILL, typename V>
+__global__ __launch_bounds__(256) void bcrs_build(BcrsBuild<V> a) {
+    extern __shared__ int32_t tile_col[];
+    V *tile_val = reinterpret_cast<V *>(tile_col + a.cap);
+    constexpr int kRows = 256 / GS;
+    const int64_t br0 = static_cast<int64_t>(blockIdx.x) * kRows;
+    const int64_t br1 = min<int64_t>(br0 + kRows, a.nbr);
+    const int64_t br = br0 + static_cast<int>(threadIdx.x) / GS;
+    const bool row_valid = br < br1;
+    const int32_t base = a.row_ptr[br0 * a.b_h];
+    const int32_t len = a.row_ptr[br1 * a.b_h] - base;
+    if (len >= 0 && len <= a.cap) {
+        for (int32_t t = threadIdx.x; t < len; t += 256) {
+            tile_col[t] = a.col[base + t];
+            if (FILL) tile_val[t] = a.val[base + t];
+        }
+        __syncthreads();
+        merge_block_row<GS, FILL, true>(a, br, row_valid, Entries<true, int32_t>{tile_col, base},
+                                        Entries<true, V>{tile_val, base});
     } else {
-        cnt[br] = u;
+        merge_block_row<GS, FILL, false>(a, br, row_valid, Entries<false, int32_t>{a.col, 0},
+                                         Entries<false, V>{a.val, 0});
     }
-}
-
-template <int GS>
-__global__ __launch_bounds__(256) void bcrs_fill_cols(int64_t nbr, int b_h, int b_w,
-                                                      const int32_t *__restrict__ row_ptr,
-                                                      const int32_t *__restrict__ col,
-                                                      const int64_t *__restrict__ incl,
-                                                      int32_t *__restrict__ brow_ptr,
-                                                      int32_t *__restrict__ bcol) {
-    const int64_t br = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / GS;
-    const bool valid = br < nbr;
-    const int64_t start = valid ? (br == 0 ? 0 : incl[br - 1]) : 0;
-    if (valid && (threadIdx.x & (GS - 1)) == 0) {
-        brow_ptr[br + 1] = static_cast<int32_t>(incl[br]);
-        if (br == 0) brow_ptr[0] = 0;
-    }
-    merge_block_row<GS>(row_ptr, col, br, valid, b_h, b_w, bcol + start);
 }
 
 // Lanes per block row of the conversion: the power of two >= b_h (<= 64).
@@ -387,28 +471,38 @@ inline int bcrs_group(int64_t b_h) {
     return g;
 }
 
-// Places every CRS value in its block: one thread per CRS row walks the row
-// and its block row's (sorted) block columns together; the row's block
-// columns are non-decreasing (checked by bcrs_count), so each entry's block
-// is found by stepping forward.
-template <typename V>
-__global__ void bcrs_fill_vals(int64_t n_rows, int b_h, int b_w, int h_pad, int w_pad,
-                               int col_major, const int32_t *__restrict__ row_ptr,
-                               const int32_t *__restrict__ col, const V *__restrict__ val,
-                               const int32_t *__restrict__ brow_ptr,
-                               const int32_t *__restrict__ bcol, V *__restrict__ bval) {
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= n_rows) return;
-    const int64_t br = r / b_h, lr = r - br * b_h;
-    int64_t a = brow_ptr[br];
-    const int64_t bs = static_cast<int64_t>(h_pad) * w_pad;
-    for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; ++j) {
-        const int32_t c = col[j], bc = c / b_w;
-        while (bcol[a] < bc) ++a;
-        const int64_t inc = col_major ? static_cast<int64_t>(h_pad) * (c % b_w) + lr
-                                      : static_cast<int64_t>(w_pad) * lr + (c % b_w);
-        bval[a * bs + inc] = val[j];
+// LDS tile of a pass: room for 1.25x a workgroup's average entries (plus
+// slack), at most 64 KB.
+inline int bcrs_tile_entries(int64_t nnz, int64_t n_wg, int64_t entry_bytes) {
+    const int64_t avg = nnz / (n_wg > 0 ? n_wg : 1);
+    const int64_t most = (65536 / entry_bytes) / 256 * 256;
+    const int64_t want = (avg + avg / 4 + 256 + 255) / 256 * 256;
+    return static_cast<int>(want < most ? want : most);
+}
+
+template <bool FILL, typename V>
+static int launch_bcrs_build(smax_hip_ctx ctx, BcrsBuild<V> a, int64_t nnz) {
+    const int gs = bcrs_group(a.b_h);
+    const int64_t n_wg = (a.nbr + 256 / gs - 1) / (256 / gs);
+    const int64_t entry = 4 + (FILL ? sizeof(V) : 0);
+    a.cap = bcrs_tile_entries(nnz, n_wg, entry);
+    const size_t lds = static_cast<size_t>(a.cap) * entry;
+    switch (gs) {
+#define SMAX_BCRS_BUILD(G)                                                                \
+    case G:                                                                               \
+        hipLaunchKernelGGL((bcrs_build<G, FILL, V>), dim3(static_cast<unsigned>(n_wg)),   \
+                           dim3(256), lds, ctx->stream, a);                               \
+        break;
+        SMAX_BCRS_BUILD(1)
+        SMAX_BCRS_BUILD(2)
+        SMAX_BCRS_BUILD(4)
+        SMAX_BCRS_BUILD(8)
+        SMAX_BCRS_BUILD(16)
+        SMAX_BCRS_BUILD(32)
+        SMAX_BCRS_BUILD(64)
+#undef SMAX_BCRS_BUILD
     }
+    return check_launch(ctx, FILL ? "bcrs_build(fill)" : "bcrs_build(count)");
 }
 
 static int check_bcrs(smax_hip_ctx ctx, const char *fn, int value_type,
@@ -558,32 +652,35 @@ int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
     if (!rc)
         rc = dev_alloc(ctx, reinterpret_cast<void **>(&tmp),
                        8 * scan_tmp_elems(nbr), "bcrs");
-    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&bad), 8, "bcrs");
-    if (!rc) rc = smax_hip_zero(ctx, bad, 8);
-    int64_t n_blocks = 0, nbad = 0;
-    if (!rc) {
-#define SMAX_BCRS_COUNT(GS)                                                              \
-    hipLaunchKernelGGL(bcrs_count<GS>, grid_for(nbr * GS, 256), dim3(256), 0, ctx->stream, \
-                       nbr, static_cast<int>(b_h), static_cast<int>(b_w), row_ptr, col, cnt, bad)
-        switch (bcrs_group(b_h)) {
-        case 1: SMAX_BCRS_COUNT(1); break;
-        case 2: SMAX_BCRS_COUNT(2); break;
-        case 4: SMAX_BCRS_COUNT(4); break;
-        case 8: SMAX_BCRS_COUNT(8); break;
-        case 16: SMAX_BCRS_COUNT(16); break;
-        case 32: SMAX_BCRS_COUNT(32); break;
-        default: SMAX_BCRS_COUNT(64); break;
-        }
-#undef SMAX_BCRS_COUNT
-        rc = check_launch(ctx, "bcrs_count");
-    }
+    if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&bad), 16, "bcrs");
+    if (!rc) rc = smax_hip_zero(ctx, bad, 16);
+    BcrsBuild<uint32_t> args{};
+    args.nbr = nbr;
+    args.n_cols = n_cols;
+    args.b_h = static_cast<int>(b_h);
+    args.b_w = static_cast<int>(b_w);
+    args.h_pad = static_cast<int>(h_pad);
+    args.w_pad = static_cast<int>(w_pad);
+    args.col_major = block_col_major ? 1 : 0;
+    args.div = block_div(b_w);
+    args.row_ptr = row_ptr;
+    args.col = col;
+    args.cnt = cnt;
+    args.bad = bad;
+    int64_t n_blocks = 0, nbad[2] = {0, 0};
+    if (!rc) rc = launch_bcrs_build<false>(ctx, args, nnz);
     if (!rc) rc = scan_inclusive<int64_t, false>(ctx, cnt, incl, nbr, tmp);
-    if (!rc) rc = read_i64(ctx, bad, &nbad);
+    if (!rc) rc = read_i64(ctx, bad, &nbad[0]);
+    if (!rc) rc = read_i64(ctx, bad + 1, &nbad[1]);
     if (!rc) rc = read_i64(ctx, incl + nbr - 1, &n_blocks);
-    if (!rc && nbad)
+    if (!rc && nbad[1])
+        rc = fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                  "smax_hip_bcrs_from_crs: %lld block rows hold column indices outside "
+                  "[0, %lld)", (long long)nbad[1], (long long)n_cols);
+    if (!rc && nbad[0])
         rc = fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
                   "Blocked Column array does not fit with crs columns "
-                  "(%lld block rows with descending columns)", (long long)nbad);
+                  "(%lld block rows with descending columns)", (long long)nbad[0]);
     if (!rc && n_blocks * h_pad * w_pad > INT32_MAX * 4LL)
         rc = fail(ctx, SMAX_HIP_ERROR_OVERFLOW, "bcrs: too many blocks");
     if (!rc)
@@ -597,45 +694,32 @@ int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
                        "bcrs val");
     if (!rc) rc = smax_hip_zero(ctx, info.val, val_bytes * n_blocks * h_pad * w_pad);
     if (!rc) {
-#define SMAX_BCRS_FILL(GS)                                                                 \
-    hipLaunchKernelGGL(bcrs_fill_cols<GS>, grid_for(nbr * GS, 256), dim3(256), 0, ctx->stream, \
-                       nbr, static_cast<int>(b_h), static_cast<int>(b_w), row_ptr, col, incl,  \
-                       info.row_ptr, info.col)
-        switch (bcrs_group(b_h)) {
-        case 1: SMAX_BCRS_FILL(1); break;
-        case 2: SMAX_BCRS_FILL(2); break;
-        case 4: SMAX_BCRS_FILL(4); break;
-        case 8: SMAX_BCRS_FILL(8); break;
-        case 16: SMAX_BCRS_FILL(16); break;
-        case 32: SMAX_BCRS_FILL(32); break;
-        default: SMAX_BCRS_FILL(64); break;
-        }
-#undef SMAX_BCRS_FILL
-        rc = check_launch(ctx, "bcrs_fill_cols");
-    }
-    if (!rc && nnz > 0) {
-        const int bh = static_cast<int>(b_h), bw = static_cast<int>(b_w);
-        const int hp = static_cast<int>(h_pad), wp = static_cast<int>(w_pad);
-        const int cm = block_col_major ? 1 : 0;
+        auto fill = [&](auto *vals) {
+            using V = std::remove_const_t<std::remove_pointer_t<decltype(vals)>>;
+            BcrsBuild<V> f{};
+            f.nbr = args.nbr;
+            f.n_cols = args.n_cols;
+            f.b_h = args.b_h;
+            f.b_w = args.b_w;
+            f.h_pad = args.h_pad;
+            f.w_pad = args.w_pad;
+            f.col_major = args.col_major;
+            f.div = args.div;
+            f.row_ptr = row_ptr;
+            f.col = col;
+            f.val = vals;
+            f.incl = incl;
+            f.brow_ptr = info.row_ptr;
+            f.bcol = info.col;
+            f.bval = static_cast<V *>(info.val);
+            return launch_bcrs_build<true>(ctx, f, nnz);
+        };
         if (val_bytes == 8)
-            hipLaunchKernelGGL(bcrs_fill_vals<uint64_t>, grid_for(n_rows, 256), dim3(256), 0,
-                               ctx->stream, n_rows, bh, bw, hp, wp, cm,
-                               row_ptr, col, static_cast<const uint64_t *>(val),
-                               info.row_ptr, info.col,
-                               static_cast<uint64_t *>(info.val));
+            rc = fill(static_cast<const uint64_t *>(val));
         else if (val_bytes == 4)
-            hipLaunchKernelGGL(bcrs_fill_vals<uint32_t>, grid_for(n_rows, 256), dim3(256), 0,
-                               ctx->stream, n_rows, bh, bw, hp, wp, cm,
-                               row_ptr, col, static_cast<const uint32_t *>(val),
-                               info.row_ptr, info.col,
-                               static_cast<uint32_t *>(info.val));
+            rc = fill(static_cast<const uint32_t *>(val));
         else
-            hipLaunchKernelGGL(bcrs_fill_vals<uint16_t>, grid_for(n_rows, 256), dim3(256), 0,
-                               ctx->stream, n_rows, bh, bw, hp, wp, cm,
-                               row_ptr, col, static_cast<const uint16_t *>(val),
-                               info.row_ptr, info.col,
-                               static_cast<uint16_t *>(info.val));
-        rc = check_launch(ctx, "bcrs_fill_vals");
+            rc = fill(static_cast<const uint16_t *>(val));
     }
     (void)hipStreamSynchronize(ctx->stream);
     dev_free(cnt);

@@ -172,6 +172,53 @@ def test_bcrs_conversion_duplicate_entries_last_wins(ctx):
     b.destroy()
 
 
+def mixed_row_lengths(n, short, long_from, long_len, seed):
+    """n x n CRS: rows below long_from hold `short` ascending columns near the
+    diagonal, the others long_len distinct ascending random columns."""
+    rng = np.random.default_rng(seed)
+    cols = []
+    for r in range(n):
+        if r < long_from:
+            c = np.clip(np.arange(r - short // 2, r - short // 2 + short), 0, n - 1)
+            cols.append(np.unique(c))
+        else:
+            cols.append(np.sort(rng.choice(n, long_len, replace=False)))
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in cols])
+    col = np.concatenate(cols).astype(np.int32)
+    return rp, col, rng.uniform(-1.0, 1.0, int(rp[-1]))
+
+
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("cm", [True, False])
+def test_bcrs_conversion_long_rows_read_from_memory(ctx, vt, cm):
+    """Workgroups whose CRS range exceeds the LDS tile merge straight from
+    memory: rows 0..511 are short (staged), rows 512..1023 hold 600 entries
+    each (256 such rows per workgroup, far past the tile)."""
+    rp, col, v64 = mixed_row_lengths(1024, 3, 512, 600, 11)
+    val = vals_for(vt, v64)
+    b = gpu_bcrs(ctx, 1024, 1024, rp, col, val, 4, 4, 4, 4, cm)
+    want = oracle.crs_to_bcrs(1024, 1024, rp, col, val, 4, 4, 4, 4, cm)
+    got = b.arrays(val.dtype)
+    assert b.info.n_blocks == want["n_blocks"]
+    for f in ("col", "row_ptr"):
+        assert_bitwise(got[f], want[f], f)
+    assert_bitwise(got["val"].view(np.uint8), want["val"].view(np.uint8), "val")
+    b.destroy()
+
+
+def test_bcrs_conversion_rejects_columns_out_of_range(ctx):
+    """A column index outside [0, n_cols) would place a value outside its
+    block: the conversion refuses the matrix."""
+    rp = np.array([0, 2, 3, 4, 4], np.int32)
+    for bad in (4, -1, 1 << 30):
+        col = np.array([0, 1, bad, 2], np.int32)
+        with pytest.raises(capi.SmaxHipError) as e:
+            gpu_bcrs(ctx, 4, 4, rp, col, np.ones(4), 2, 2, 2, 2, False)
+        assert e.value.status == capi.ERROR_INVALID_VALUE
+        assert "outside" in str(e.value)
+
+
 def test_bcrs_conversion_rejects_what_the_reference_rejects(ctx):
     A = matgen.poisson2d(6)  # 36 x 36
     for bh, bw, hp, wp in ((5, 4, 5, 4), (4, 5, 4, 5), (4, 4, 3, 4), (4, 4, 4, 3)):
