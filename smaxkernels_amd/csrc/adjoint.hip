@@ -96,7 +96,57 @@ __global__ __launch_bounds__(256) void scs_sddmm_kernel(
 // chain over the row's elements is software-pipelined (the next U column and
 // src indices are loaded before this group's X rows), and the U results of a
 // group, consecutive CRS positions of one row, leave in one store
-// instruction from U lanes (GL / U instructions when GL < U).
+// instruction from U lanes (GL / U instructions when GL < U).  With GL >= U
+// the U elements of a group share one butterfly (group_reduce_scatter) that
+// ends with each element's sum on the lane that stores it, and each lane
+// loads only the src index it stores to.
+// The same butterfly over GL >= U lanes for U elements at once: at each
+// offset (GL/2 first) a lane keeps the half of its live elements that its
+// lane bit selects and sends the other half to its partner, adding (own +
+// partner) as group_sum does, so every element's sum is the same tree.  After
+// log2(U) halving steps the remaining offsets (below GL/U) add as in
+// group_sum.  Lane l returns the sum of element l / (GL / U).
+// The halves are vector values, not array elements: a choice between two
+// elements of one array becomes an array read at a lane-dependent index,
+// which the compiler expands into a select over every element.
+template <int N, int OFF, int GL, typename A>
+__device__ __forceinline__ A scatter_steps(const A (&cur)[N], int l) {
+    if constexpr (N == 1) {
+        return cur[0];
+    } else {
+        constexpr int H = N / 2;
+        const bool upper = (l & OFF) != 0;
+        A next[H];
+        if constexpr (H == 1) {
+            const A keep = upper ? cur[1] : cur[0];
+            const A send = upper ? cur[0] : cur[1];
+            next[0] = keep + __shfl_xor(send, OFF, GL);
+        } else {
+            typedef A half_t __attribute__((ext_vector_type(H)));
+            half_t lo, hi;
+#pragma unroll
+            for (int i = 0; i < H; ++i) {
+                lo[i] = cur[i];
+                hi[i] = cur[H + i];
+            }
+            const half_t keep = upper ? hi : lo;
+            const half_t send = upper ? lo : hi;
+#pragma unroll
+            for (int i = 0; i < H; ++i) next[i] = keep[i] + __shfl_xor(send[i], OFF, GL);
+        }
+        return scatter_steps<H, OFF / 2, GL>(next, l);
+    }
+}
+
+template <int GL, int U, typename A>
+__device__ __forceinline__ A group_reduce_scatter(A (&cur)[U], int l) {
+    static_assert(GL >= U, "one element per lane at most");
+    A s = scatter_steps<U, GL / 2, GL>(cur, l);
+#pragma unroll
+    for (int off = GL / U / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, GL);
+    return s;
+}
+
 template <typename A, int U>
 __device__ __forceinline__ A pick(const A (&r)[U], int idx) {
     A out = r[0];
@@ -143,7 +193,45 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
     A g[Q];
     load_run<A, Q>(Gm + static_cast<int64_t>(r) * K + l * Q, g);
     int j = 0;
-    if (len >= U) {
+    if (GL >= U && len >= U) {
+        // lane l stores element l / (GL / U) of each group of U
+        constexpr int kPer = GL >= U ? GL / U : 1;
+        const int mine = l / kPer;
+        const bool stores = l % kPer == 0;
+        int32_t c0[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) c0[u] = col[base + u * C];
+        int32_t s0 = src[base + static_cast<int64_t>(mine) * C];
+        for (;;) {
+            const bool more = j + 2 * U <= len;
+            int32_t c1[U], s1 = 0;
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    c1[u] = col[base + static_cast<int64_t>(j + U + u) * C];
+                s1 = src[base + static_cast<int64_t>(j + U + mine) * C];
+            }
+            A xv[U][Q];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                load_run<A, Q>(X + static_cast<int64_t>(c0[u]) * K + l * Q, xv[u]);
+            A part[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                A acc = 0;
+#pragma unroll
+                for (int t = 0; t < Q; ++t) acc = T::fma_(g[t], xv[u][t], acc);
+                part[u] = acc;
+            }
+            const A sum = group_reduce_scatter<(GL >= U ? GL : U), U>(part, l);
+            if (stores) dvals[s0] = sum;
+            j += U;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) c0[u] = c1[u];
+            s0 = s1;
+        }
+    } else if (len >= U) {
         int32_t c0[U], s0[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
