@@ -96,10 +96,10 @@ __global__ __launch_bounds__(256) void scs_sddmm_kernel(
 // chain over the row's elements is software-pipelined (the next U column and
 // src indices are loaded before this group's X rows), and the U results of a
 // group, consecutive CRS positions of one row, leave in one store
-// instruction from U lanes (GL / U instructions when GL < U).  With GL >= U
-// the U elements of a group share one butterfly (group_reduce_scatter) that
-// ends with each element's sum on the lane that stores it, and each lane
-// loads only the src index it stores to.
+// instruction from U lanes (U / GL instructions when GL < U).  The U
+// elements of a group share one butterfly (group_reduce_scatter; per GL
+// elements when GL < U) that ends with each element's sum on the lane that
+// stores it, and each lane loads only the src indices it stores to.
 // The same butterfly over GL >= U lanes for U elements at once: at each
 // offset (GL/2 first) a lane keeps the half of its live elements that its
 // lane bit selects and sends the other half to its partner, adding (own +
@@ -147,15 +147,6 @@ __device__ __forceinline__ A group_reduce_scatter(A (&cur)[U], int l) {
     return s;
 }
 
-template <typename A, int U>
-__device__ __forceinline__ A pick(const A (&r)[U], int idx) {
-    A out = r[0];
-#pragma unroll
-    for (int u = 1; u < U; ++u)
-        if (idx == u) out = r[u];
-    return out;
-}
-
 template <int VT, int K>
 __global__ __launch_bounds__(256) void scs_sddmm_rows(
     int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
@@ -193,23 +184,29 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
     A g[Q];
     load_run<A, Q>(Gm + static_cast<int64_t>(r) * K + l * Q, g);
     int j = 0;
-    if (GL >= U && len >= U) {
-        // lane l stores element l / (GL / U) of each group of U
+    if (len >= U) {
+        // kPer lanes hold each result, each lane holds kNpl results: lane l
+        // stores elements n*GL + l/kPer (n < kNpl) of each group of U
         constexpr int kPer = GL >= U ? GL / U : 1;
-        const int mine = l / kPer;
+        constexpr int kNpl = GL >= U ? 1 : U / GL;
+        const int lane_elem = l / kPer;
         const bool stores = l % kPer == 0;
-        int32_t c0[U];
+        int32_t c0[U], s0[kNpl];
 #pragma unroll
         for (int u = 0; u < U; ++u) c0[u] = col[base + u * C];
-        int32_t s0 = src[base + static_cast<int64_t>(mine) * C];
+#pragma unroll
+        for (int n = 0; n < kNpl; ++n)
+            s0[n] = src[base + static_cast<int64_t>(n * GL + lane_elem) * C];
         for (;;) {
             const bool more = j + 2 * U <= len;
-            int32_t c1[U], s1 = 0;
+            int32_t c1[U], s1[kNpl];
             if (more) {
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                     c1[u] = col[base + static_cast<int64_t>(j + U + u) * C];
-                s1 = src[base + static_cast<int64_t>(j + U + mine) * C];
+#pragma unroll
+                for (int n = 0; n < kNpl; ++n)
+                    s1[n] = src[base + static_cast<int64_t>(j + U + n * GL + lane_elem) * C];
             }
             A xv[U][Q];
 #pragma unroll
@@ -223,55 +220,24 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
                 for (int t = 0; t < Q; ++t) acc = T::fma_(g[t], xv[u][t], acc);
                 part[u] = acc;
             }
-            const A sum = group_reduce_scatter<(GL >= U ? GL : U), U>(part, l);
-            if (stores) dvals[s0] = sum;
+            if constexpr (GL >= U) {
+                const A sum = group_reduce_scatter<GL, U>(part, l);
+                if (stores) dvals[s0[0]] = sum;
+            } else {
+#pragma unroll
+                for (int n = 0; n < kNpl; ++n) {
+                    A piece[GL];
+#pragma unroll
+                    for (int i = 0; i < GL; ++i) piece[i] = part[n * GL + i];
+                    dvals[s0[n]] = group_reduce_scatter<GL, GL>(piece, l);
+                }
+            }
             j += U;
             if (!more) break;
 #pragma unroll
             for (int u = 0; u < U; ++u) c0[u] = c1[u];
-            s0 = s1;
-        }
-    } else if (len >= U) {
-        int32_t c0[U], s0[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            c0[u] = col[base + u * C];
-            s0[u] = src[base + u * C];
-        }
-        for (;;) {
-            const bool more = j + 2 * U <= len;
-            int32_t c1[U], s1[U];
-            if (more) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    c1[u] = col[base + static_cast<int64_t>(j + U + u) * C];
-                    s1[u] = src[base + static_cast<int64_t>(j + U + u) * C];
-                }
-            }
-            A xv[U][Q];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                load_run<A, Q>(X + static_cast<int64_t>(c0[u]) * K + l * Q, xv[u]);
-            A res[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                A acc = 0;
-#pragma unroll
-                for (int t = 0; t < Q; ++t) acc = T::fma_(g[t], xv[u][t], acc);
-                res[u] = group_sum<GL>(acc);
-            }
-#pragma unroll
-            for (int w = 0; w < U; w += GL) {
-                const int idx = w + l;
-                if (idx < U) dvals[pick<int32_t, U>(s0, idx)] = pick<A, U>(res, idx);
-            }
-            j += U;
-            if (!more) break;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                c0[u] = c1[u];
-                s0[u] = s1[u];
-            }
+            for (int n = 0; n < kNpl; ++n) s0[n] = s1[n];
         }
     }
     for (; j < len; ++j) {

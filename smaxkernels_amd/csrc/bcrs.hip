@@ -269,21 +269,25 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
     const dim3 g = grid_for(n_block_rows * (runs ? k / Q : k), kBThreads);
     // block groups: 8 when a single thread holds the whole block row (k = 1),
     // else 2 (measured on the 96^3 block matrix: k = 1 0.108 -> 0.058 ms,
-    // k = 8 0.173 -> 0.160 ms, k = 64 0.663 -> 0.649 ms in BF16)
-    const bool single = (runs ? k / Q : k) == 1;
+    // k = 8 0.173 -> 0.160 ms, k = 64 0.663 -> 0.649 ms in BF16).  Padded
+    // blocks load value by value, with an address each: groups of 8 would
+    // spill registers, so a single thread takes kPaddedGroup blocks at once.
+    constexpr int kPaddedGroup = 4;
+    const bool single = !runs && k == 1;
 #define SMAX_BCRS_REG_LAUNCH(QQ, DD, CC, GG)                                                \
     hipLaunchKernelGGL((bcrs_spmm_reg<VT, BH, BW, QQ, DD, CC, GG>), g, dim3(kBThreads), 0,  \
                        ctx->stream, n_block_rows, static_cast<int>(h_pad),                 \
                        static_cast<int>(w_pad), cm ? 1 : 0, k, col, row_ptr, v, xv, yv)
-#define SMAX_BCRS_REG_G(QQ, DD, CC)                                                         \
-    if (single) SMAX_BCRS_REG_LAUNCH(QQ, DD, CC, 8);                                        \
-    else SMAX_BCRS_REG_LAUNCH(QQ, DD, CC, 2)
-    if (runs && dense && cm) { SMAX_BCRS_REG_G(Q, true, true); }
-    else if (runs && dense) { SMAX_BCRS_REG_G(Q, true, false); }
-    else if (runs) { SMAX_BCRS_REG_G(Q, false, false); }
-    else if (dense && cm) { SMAX_BCRS_REG_G(1, true, true); }
-    else if (dense) { SMAX_BCRS_REG_G(1, true, false); }
-    else { SMAX_BCRS_REG_G(1, false, false); }
+#define SMAX_BCRS_REG_G(DD, CC, GS)                                                         \
+    if (single) SMAX_BCRS_REG_LAUNCH(1, DD, CC, GS);                                        \
+    else SMAX_BCRS_REG_LAUNCH(1, DD, CC, 2)
+    // runs imply k >= 4Q lanes' worth of columns, so never a single thread
+    if (runs && dense && cm) { SMAX_BCRS_REG_LAUNCH(Q, true, true, 2); }
+    else if (runs && dense) { SMAX_BCRS_REG_LAUNCH(Q, true, false, 2); }
+    else if (runs) { SMAX_BCRS_REG_LAUNCH(Q, false, false, 2); }
+    else if (dense && cm) { SMAX_BCRS_REG_G(true, true, 8); }
+    else if (dense) { SMAX_BCRS_REG_G(true, false, 8); }
+    else { SMAX_BCRS_REG_G(false, false, kPaddedGroup); }
 #undef SMAX_BCRS_REG_G
 #undef SMAX_BCRS_REG_LAUNCH
 }

@@ -62,7 +62,8 @@ def sddmm_tree(k):
 
 
 @pytest.mark.parametrize("vt,k", [(capi.F32, 32), (capi.F64, 32), (capi.F32, 8),
-                                  (capi.BF16, 64), (capi.F64, 2)])
+                                  (capi.BF16, 64), (capi.F64, 2), (capi.F64, 16),
+                                  (capi.F32, 4), (capi.F64, 64), (capi.F32, 1)])
 def test_sddmm_row_and_element_kernels_agree(ctx, vt, k):
     """With X 16-byte aligned a power-of-two k takes the row kernel (16-byte
     loads); shifted by one entry it takes the per-element kernel.  Both
