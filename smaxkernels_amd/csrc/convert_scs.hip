@@ -138,30 +138,43 @@ __device__ __forceinline__ int32_t suffix_at(const int32_t *SR, int64_t q,
 }
 
 // Stopper ranks, position lists and the swap count s of every segment.
-__global__ void k_rank(int64_t n, const int32_t *__restrict__ head,
-                       const int32_t *__restrict__ state,
-                       const int32_t *__restrict__ end,
-                       const int32_t *__restrict__ cnt,
-                       const int32_t *__restrict__ pv,
-                       const int32_t *__restrict__ PL,
-                       const int32_t *__restrict__ SR,
-                       int32_t *__restrict__ posL, int32_t *__restrict__ posR,
-                       int32_t *__restrict__ sw) {
-    int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const int64_t h = head[q];
-    if (state[h] != SEG_ACTIVE) return;
-    const int64_t e = end[h];
-    const int32_t se = suffix_at(SR, e, n);
-    const int32_t Lq = PL[q] - PL[h];
-    if (q > h) {
-        const int32_t c = cnt[q], p = pv[h];
-        if (c <= p) posL[h + Lq] = static_cast<int32_t>(q);
-        if (c >= p) posR[h + (SR[q] - se)] = static_cast<int32_t>(q);
+// s is a maximum over the segment's positions: a wavefront whose lanes all
+// lie in one segment takes its maximum first and issues one atomic (one
+// per element would serialise millions of atomics on one address when a
+// sigma window is one large segment).
+__global__ __launch_bounds__(256) void k_rank(
+    int64_t n, const int32_t *__restrict__ head, const int32_t *__restrict__ state,
+    const int32_t *__restrict__ end, const int32_t *__restrict__ cnt,
+    const int32_t *__restrict__ pv, const int32_t *__restrict__ PL,
+    const int32_t *__restrict__ SR, int32_t *__restrict__ posL,
+    int32_t *__restrict__ posR, int32_t *__restrict__ sw) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t h = -1;
+    int32_t cand = 0;
+    if (q < n) {
+        h = head[q];
+        if (state[h] == SEG_ACTIVE) {
+            const int64_t e = end[h];
+            const int32_t se = suffix_at(SR, e, n);
+            const int32_t Lq = PL[q] - PL[h];
+            if (q > h) {
+                const int32_t c = cnt[q], p = pv[h];
+                if (c <= p) posL[h + Lq] = static_cast<int32_t>(q);
+                if (c >= p) posR[h + (SR[q] - se)] = static_cast<int32_t>(q);
+            }
+            const int32_t Rq1 = q + 1 < e ? SR[q + 1] - se : 0;
+            cand = Lq < Rq1 ? Lq : Rq1;
+        }
     }
-    const int32_t Rq1 = q + 1 < e ? SR[q + 1] - se : 0;
-    const int32_t cand = Lq < Rq1 ? Lq : Rq1;
-    if (cand > 0) atomicMax(sw + h, cand);
+    // every lane of the wavefront reaches this point
+    if (__all(h == __shfl(h, 0))) {
+        int32_t m = cand;
+#pragma unroll
+        for (int off = kWave / 2; off >= 1; off >>= 1) m = max(m, __shfl_xor(m, off));
+        if ((threadIdx.x & (kWave - 1)) == 0 && m > 0) atomicMax(sw + h, m);
+    } else if (cand > 0) {
+        atomicMax(sw + h, cand);
+    }
 }
 
 __global__ void k_cut(int64_t nl, const int32_t *__restrict__ list,
