@@ -236,7 +236,6 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
             if (!more) break;
 #pragma unroll
             for (int u = 0; u < U; ++u) c0[u] = c1[u];
-#pragma unroll
             for (int n = 0; n < kNpl; ++n) s0[n] = s1[n];
         }
     }
