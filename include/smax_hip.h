@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SMAX_HIP_ABI_VERSION 1
+#define SMAX_HIP_ABI_VERSION 2
 
 /* ------------------------------------------------------------------ status */
 typedef enum {
@@ -398,6 +398,19 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                        int64_t k, const void *G, const void *X, void *dvals);
 
 /* ----------------------------------------------------------- CG solvers */
+/* Column codings the solvers may use for their slot-space matrix.  Every
+ * coding gives each element the same column, so the iterates do not depend on
+ * the choice; narrower ones move fewer index bytes per product. */
+typedef enum {
+    SMAX_HIP_COLUMNS_AUTO = 0, /* the narrowest that fits: one-byte indices
+                                  into per-chunk tables of slot offsets (C =
+                                  64, at most 32 distinct offsets per chunk),
+                                  else 16-bit offsets from the slot, else
+                                  32-bit columns                            */
+    SMAX_HIP_COLUMNS_32 = 1,   /* 32-bit slot-space columns                  */
+    SMAX_HIP_COLUMNS_16 = 2    /* 16-bit offsets when they fit, else 32-bit  */
+} smax_hip_column_coding;
+
 typedef struct {
     double rel_tol;         /* stop when ||b - A x||_2 / ||b||_2 <= rel_tol */
     int64_t max_iters;      /* total (inner) iteration cap                   */
@@ -413,6 +426,7 @@ typedef struct {
                                across refinement steps (reliable updates),
                                1 = restart CG on each correction equation
                                A d = r (classical iterative refinement)     */
+    int32_t column_coding;  /* smax_hip_column_coding (0: narrowest)        */
 } smax_hip_cg_params;
 
 typedef struct {

@@ -58,7 +58,8 @@ class CgParams(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("max_iters", i64),
                 ("check_every", i64), ("inner_rel_tol", C.c_double),
                 ("max_inner", i64), ("inner_value_type", C.c_int32),
-                ("keep_diag_f32", C.c_int32), ("restart_inner", C.c_int32)]
+                ("keep_diag_f32", C.c_int32), ("restart_inner", C.c_int32),
+                ("column_coding", C.c_int32)]
 
 
 class CgReport(C.Structure):

@@ -117,25 +117,53 @@ __device__ __forceinline__ float fma_w(float a, float b, float c) {
     return __builtin_fmaf(a, b, c);
 }
 
+// Column codings of the solver matrix (every one yields the same column for
+// every element, so the products are identical bit for bit):
+//   kCol32:  32-bit slot-space columns;
+//   kRel16:  16-bit offsets from the slot (rel[e] = col_slot[e] - s; padding
+//            elements hold 0 and read the slot's own x entry, times a zero
+//            value);
+//   kTab8:   C = 64 only: a one-byte index per element into its chunk's table
+//            of kColTable 16-bit offsets; lane t of the chunk's wavefront
+//            holds entry t mod kColTable and a lane shuffle fetches the entry.
+//            A slot's indices of elements j..j+3 share one 32-bit word, so a
+//            wavefront reads the indices of four elements per 256-byte load.
+enum { kCol32 = 0, kRel16 = 1, kTab8 = 2 };
+constexpr int kColTable = 32;
+
 // (A x)[s]: sequential fma chain over the slot's SELL elements, then (DIAG)
-// the FP32 diagonal term.  REL: column indices come as 16-bit offsets from
-// the slot (rel[e] = col_slot[e] - s; padding elements hold 0 and read the
-// slot's own x entry, times a zero value).
-template <typename V, typename W, bool DIAG, bool REL>
+// the FP32 diagonal term.  With kTab8 every lane of the wavefront must run
+// this (one chunk per wavefront, all slots present).
+template <typename V, typename W, bool DIAG, int MODE>
 __device__ __forceinline__ W apply_slot(int64_t s, int64_t C,
                                         const int32_t *__restrict__ chunk_ptr,
                                         const int32_t *__restrict__ chunk_len,
                                         const int32_t *__restrict__ col,
                                         const int16_t *__restrict__ rel,
+                                        const uint32_t *__restrict__ idx,
+                                        const int64_t *__restrict__ idx_row,
+                                        const int16_t *__restrict__ tab,
                                         const V *__restrict__ val,
                                         const float *__restrict__ diag,
                                         const W *__restrict__ x) {
     const int64_t c = s / C;
     const int64_t base = static_cast<int64_t>(chunk_ptr[c]) + (s - c * C);
     const int len = chunk_len[c];
+    int entry = 0;
+    const uint32_t *iw = nullptr;  // kTab8: this slot's index words, one per row
+    if constexpr (MODE == kTab8) {
+        entry = tab[c * kColTable + (threadIdx.x & (kColTable - 1))];
+        iw = idx + idx_row[c] * kWave + (s - c * C);
+    }
+    // kTab8: the column of element j, its index byte taken from word w
+    auto tab_col = [&](uint32_t w, int j) -> int64_t {
+        return s + __shfl(entry, static_cast<int>((w >> (8 * (j & 3))) & 0xffu));
+    };
     auto col_at = [&](int64_t o) -> int64_t {
-        if constexpr (REL) return s + ld_stream(rel + o);
-        else return ld_stream(col + o);
+        if constexpr (MODE == kRel16)
+            return s + ld_stream(rel + o);
+        else
+            return ld_stream(col + o);
     };
     W acc = 0;
     constexpr int U = 8;
@@ -144,9 +172,17 @@ __device__ __forceinline__ W apply_slot(int64_t s, int64_t C,
         int64_t cc[U];
         V vv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            cc[u] = col_at(base + (j + u) * C);
-            vv[u] = ld_stream(val + base + (j + u) * C);
+        for (int u = 0; u < U; ++u) vv[u] = ld_stream(val + base + (j + u) * C);
+        if constexpr (MODE == kTab8) {
+            // j is a multiple of 8: two whole index words
+            uint32_t w[U / 4];
+#pragma unroll
+            for (int g = 0; g < U / 4; ++g) w[g] = ld_stream(iw + ((j >> 2) + g) * kWave);
+#pragma unroll
+            for (int u = 0; u < U; ++u) cc[u] = tab_col(w[u >> 2], u);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) cc[u] = col_at(base + (j + u) * C);
         }
         W xv[U];
 #pragma unroll
@@ -155,27 +191,34 @@ __device__ __forceinline__ W apply_slot(int64_t s, int64_t C,
         for (int u = 0; u < U; ++u)
             acc = fma_w(static_cast<W>(widen_t<V>::f(vv[u])), xv[u], acc);
     }
-    for (; j < len; ++j)
-        acc = fma_w(static_cast<W>(widen_t<V>::f(val[base + j * C])),
-                    x[col_at(base + j * C)], acc);
+    for (; j < len; ++j) {
+        int64_t cj;
+        if constexpr (MODE == kTab8)
+            cj = tab_col(ld_stream(iw + (j >> 2) * kWave), j);
+        else
+            cj = col_at(base + j * C);
+        acc = fma_w(static_cast<W>(widen_t<V>::f(val[base + j * C])), x[cj], acc);
+    }
     if (DIAG) acc = fma_w(static_cast<W>(diag[s]), x[s], acc);
     return acc;
 }
 
 // q = A p, partial[block] = sum p.q
-template <typename V, typename W, bool DIAG, bool REL>
+template <typename V, typename W, bool DIAG, int MODE>
 __global__ __launch_bounds__(kCgThreads) void k_spmv_dot(
     int64_t n, int64_t C, const int32_t *__restrict__ chunk_ptr,
     const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ col,
-    const int16_t *__restrict__ rel, const V *__restrict__ val,
+    const int16_t *__restrict__ rel, const uint32_t *__restrict__ idx,
+    const int64_t *__restrict__ idx_row, const int16_t *__restrict__ tab,
+    const V *__restrict__ val,
     const float *__restrict__ diag, const W *__restrict__ p, W *__restrict__ q,
     W *__restrict__ partial, const int64_t *__restrict__ ctl) {
     if (ctl[CTL_DONE]) return;
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
     W prod = 0;
     if (s < n) {
-        const W v = apply_slot<V, W, DIAG, REL>(s, C, chunk_ptr, chunk_len, col, rel, val,
-                                                diag, p);
+        const W v = apply_slot<V, W, DIAG, MODE>(s, C, chunk_ptr, chunk_len, col, rel, idx,
+                                                 idx_row, tab, val, diag, p);
         q[s] = v;
         prod = p[s] * v;
     }
@@ -184,18 +227,20 @@ __global__ __launch_bounds__(kCgThreads) void k_spmv_dot(
 }
 
 // r = b - A x, partial = sum r.r  (FP64 outer residual)
-template <bool REL>
+template <int MODE>
 __global__ __launch_bounds__(kCgThreads) void k_residual(
     int64_t n, int64_t C, const int32_t *__restrict__ chunk_ptr,
     const int32_t *__restrict__ chunk_len, const int32_t *__restrict__ col,
-    const int16_t *__restrict__ rel, const double *__restrict__ val,
+    const int16_t *__restrict__ rel, const uint32_t *__restrict__ idx,
+    const int64_t *__restrict__ idx_row, const int16_t *__restrict__ tab,
+    const double *__restrict__ val,
     const double *__restrict__ b, const double *__restrict__ x, double *__restrict__ r,
     double *__restrict__ partial) {
     const int64_t s = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;
     double sq = 0;
     if (s < n) {
-        const double ax = apply_slot<double, double, false, REL>(
-            s, C, chunk_ptr, chunk_len, col, rel, val, nullptr, x);
+        const double ax = apply_slot<double, double, false, MODE>(
+            s, C, chunk_ptr, chunk_len, col, rel, idx, idx_row, tab, val, nullptr, x);
         const double v = b[s] - ax;
         r[s] = v;
         sq = v * v;
@@ -429,6 +474,52 @@ __global__ __launch_bounds__(1024) void k_or_flags(const int32_t *__restrict__ f
     if (threadIdx.x == 0) out[0] = any;
 }
 
+// kTab8 tables (C = 64: one wavefront per chunk, lane = slot).  A chunk's
+// table lists the distinct offsets of its elements in order of first use
+// (element j before j + 1, lower lane first), padded with 0; the index bytes
+// go to the word layout of apply_slot (bytes past the chunk's length are 0).
+// bad[chunk] = 1 when the chunk uses more than kColTable offsets (the coding
+// is then not used).
+__global__ __launch_bounds__(256) void k_col_tab(int64_t n_chunks,
+                                                 const int32_t *__restrict__ chunk_ptr,
+                                                 const int32_t *__restrict__ chunk_len,
+                                                 const int16_t *__restrict__ rel,
+                                                 const int64_t *__restrict__ idx_row,
+                                                 int16_t *__restrict__ tab,
+                                                 uint8_t *__restrict__ idx_bytes,
+                                                 int32_t *__restrict__ bad) {
+    const int64_t c = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    if (c >= n_chunks) return;  // whole wavefronts
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const int64_t base = static_cast<int64_t>(chunk_ptr[c]) + lane;
+    const int len = chunk_len[c];
+    uint8_t *mine = idx_bytes + (idx_row[c] * kWave + lane) * 4;  // byte j%4 of row j/4
+    int entry = 0;  // lane t holds table entry t
+    int count = 0, over = 0;
+    for (int j = 0; j < len && !over; ++j) {
+        const int off = rel[base + static_cast<int64_t>(j) * kWave];
+        int pos = -1;
+        for (int t = 0; t < count; ++t)
+            if (__shfl(entry, t) == off) pos = t;
+        for (;;) {
+            const unsigned long long miss = __ballot(pos < 0);
+            if (miss == 0) break;
+            if (count == kColTable) {
+                over = 1;
+                break;
+            }
+            const int v = __shfl(off, __ffsll(static_cast<long long>(miss)) - 1);
+            if (lane == count) entry = v;
+            if (off == v) pos = count;
+            ++count;
+        }
+        if (!over) mine[(j >> 2) * kWave * 4 + (j & 3)] = static_cast<uint8_t>(pos);
+    }
+    for (int j = len; j & 3; ++j) mine[(j >> 2) * kWave * 4 + (j & 3)] = 0;
+    if (lane < kColTable) tab[c * kColTable + lane] = static_cast<int16_t>(entry);
+    if (lane == 0) bad[c] = over;
+}
+
 __global__ void k_round_f32(int64_t m, const double *__restrict__ v,
                             float *__restrict__ out) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -478,22 +569,28 @@ __global__ void k_split_bf16(int64_t npad, int64_t C,
 struct Mat {
     int64_t n, C;
     const int32_t *chunk_ptr, *chunk_len, *col;
-    const int16_t *rel;  // 16-bit column offsets from the slot, or nullptr
+    const int16_t *rel;   // 16-bit column offsets from the slot, or nullptr
+    const uint32_t *idx;     // kTab8 index words (with idx_row and tab), or nullptr
+    const int64_t *idx_row;
+    const int16_t *tab;
+    int mode() const { return idx ? kTab8 : rel ? kRel16 : kCol32; }
 };
 
-// q = A p with partial dots, through the matrix's column representation.
+// q = A p with partial dots, through the matrix's column coding.
 template <typename V, typename W, bool DIAG>
 static void launch_spmv_dot(smax_hip_ctx ctx, dim3 sg, const Mat &M, const V *val,
                             const float *diag, const W *p, W *q, W *partial,
                             const int64_t *ctl) {
-    if (M.rel)
-        hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG, true>), sg, dim3(kCgThreads), 0,
-                           ctx->stream, M.n, M.C, M.chunk_ptr, M.chunk_len, M.col, M.rel,
-                           val, diag, p, q, partial, ctl);
-    else
-        hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG, false>), sg, dim3(kCgThreads), 0,
-                           ctx->stream, M.n, M.C, M.chunk_ptr, M.chunk_len, M.col, M.rel,
-                           val, diag, p, q, partial, ctl);
+#define SMAX_CG_SPMV(MODE)                                                                   \
+    hipLaunchKernelGGL((k_spmv_dot<V, W, DIAG, MODE>), sg, dim3(kCgThreads), 0, ctx->stream, \
+                       M.n, M.C, M.chunk_ptr, M.chunk_len, M.col, M.rel, M.idx, M.idx_row,     \
+                       M.tab, val, diag, p, q, partial, ctl)
+    switch (M.mode()) {
+    case kTab8: SMAX_CG_SPMV(kTab8); break;
+    case kRel16: SMAX_CG_SPMV(kRel16); break;
+    default: SMAX_CG_SPMV(kCol32); break;
+    }
+#undef SMAX_CG_SPMV
 }
 
 struct Work {
@@ -586,12 +683,16 @@ static int residual(smax_hip_ctx ctx, const Mat &A, const double *val,
                     const double *b, const double *x, double *r, double *partial,
                     double *scal, int idx, double *rr) {
     const dim3 sg = grid_for(A.n, kCgThreads);
-    if (A.rel)
-        hipLaunchKernelGGL(k_residual<true>, sg, dim3(kCgThreads), 0, ctx->stream, A.n, A.C,
-                           A.chunk_ptr, A.chunk_len, A.col, A.rel, val, b, x, r, partial);
-    else
-        hipLaunchKernelGGL(k_residual<false>, sg, dim3(kCgThreads), 0, ctx->stream, A.n, A.C,
-                           A.chunk_ptr, A.chunk_len, A.col, A.rel, val, b, x, r, partial);
+#define SMAX_CG_RESIDUAL(MODE)                                                             \
+    hipLaunchKernelGGL(k_residual<MODE>, sg, dim3(kCgThreads), 0, ctx->stream, A.n, A.C,   \
+                       A.chunk_ptr, A.chunk_len, A.col, A.rel, A.idx, A.idx_row, A.tab, val, \
+                       b, x, r, partial)
+    switch (A.mode()) {
+    case kTab8: SMAX_CG_RESIDUAL(kTab8); break;
+    case kRel16: SMAX_CG_RESIDUAL(kRel16); break;
+    default: SMAX_CG_RESIDUAL(kCol32); break;
+    }
+#undef SMAX_CG_RESIDUAL
     hipLaunchKernelGGL((k_reduce<double>), dim3(1), dim3(kReduceThreads), 0, ctx->stream,
                        partial, static_cast<int64_t>(sg.x), scal + idx, nullptr, -1.0);
     SMAX_HIP_LAUNCHED(ctx, "k_residual");
@@ -656,6 +757,67 @@ static int prepare(smax_hip_ctx ctx, smax_hip_scs A, int lo_type, bool want_diag
             A->col_rel = nullptr;
         }
         A->col_rel_state = any ? -1 : 1;
+    }
+    if (A->col_tab_state == 0) {
+        // one-byte indices into per-chunk offset tables: a quarter of the
+        // 32-bit index stream, when every chunk uses few distinct offsets
+        // (a stencil's chunks use one per stencil point)
+        A->col_tab_state = -1;
+        if (A->col_rel_state == 1 && in.C == kWave && in.n_chunks > 0) {
+            const int64_t nc = in.n_chunks;
+            // word rows per chunk: ceil(len / 4); prefix sums on the host
+            std::vector<int32_t> lens(static_cast<size_t>(nc));
+            std::vector<int64_t> rows(static_cast<size_t>(nc));
+            rc = smax_hip_memcpy_d2h(ctx, lens.data(), in.chunk_lengths, 4 * nc);
+            if (rc) return rc;
+            int64_t total = 0;
+            for (int64_t k = 0; k < nc; ++k) {
+                rows[k] = total;
+                total += (lens[k] + 3) / 4;
+            }
+            int32_t *flags = nullptr;
+            int64_t *word = nullptr;
+            rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->col_idx),
+                           4 * kWave * (total > 0 ? total : 1), "cg col_idx");
+            if (!rc)
+                rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->col_idx_row), 8 * nc,
+                               "cg col_idx");
+            if (!rc)
+                rc = dev_alloc(ctx, reinterpret_cast<void **>(&A->col_tab),
+                               2 * kColTable * nc, "cg col_tab");
+            if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&flags), 4 * nc, "cg col_tab");
+            if (!rc) rc = dev_alloc(ctx, reinterpret_cast<void **>(&word), 8, "cg col_tab");
+            if (!rc)
+                rc = smax_hip_memcpy_h2d(ctx, A->col_idx_row, rows.data(), 8 * nc);
+            int64_t any = 1;
+            if (!rc) {
+                hipLaunchKernelGGL(k_col_tab, grid_for(nc * kWave, 256), dim3(256), 0,
+                                   ctx->stream, nc, in.chunk_ptr, in.chunk_lengths,
+                                   A->col_rel, A->col_idx_row, A->col_tab,
+                                   reinterpret_cast<uint8_t *>(A->col_idx), flags);
+                hipLaunchKernelGGL(k_or_flags, dim3(1), dim3(1024), 0, ctx->stream, flags,
+                                   nc, word);
+                rc = check_launch(ctx, "k_col_tab");
+                if (!rc) rc = read_words(ctx, word, &any, 1);
+            }
+            dev_free(flags);
+            dev_free(word);
+            if (rc || any) {
+                dev_free(A->col_idx);
+                dev_free(A->col_idx_row);
+                dev_free(A->col_tab);
+                A->col_idx = nullptr;
+                A->col_idx_row = nullptr;
+                A->col_tab = nullptr;
+                if (rc) {
+                    A->col_tab_state = 0;
+                    return rc;
+                }
+            } else {
+                A->col_tab_state = 1;
+                A->col_idx_rows = total;
+            }
+        }
     }
     const double *v64 = static_cast<const double *>(in.val);
     if (lo_type == SMAX_HIP_F32 && !A->val_f32) {
@@ -831,10 +993,19 @@ static int check_solver_args(smax_hip_ctx ctx, const char *fn, smax_hip_scs A,
     return SMAX_HIP_SUCCESS;
 }
 
-static Mat slot_matrix(smax_hip_scs A) {
+// The solver matrix in slot space with the narrowest column coding built for
+// it, limited by params->column_coding (SMAX_HIP_COLUMNS_*).
+static Mat slot_matrix(smax_hip_scs A, int32_t coding) {
     const smax_hip_scs_info &in = A->info;
-    return Mat{in.n_rows_padded, in.C, in.chunk_ptr, in.chunk_lengths,
-               A->col_slot ? A->col_slot : in.col, A->col_rel};
+    Mat M{in.n_rows_padded, in.C, in.chunk_ptr, in.chunk_lengths,
+          A->col_slot ? A->col_slot : in.col, nullptr, nullptr, nullptr, nullptr};
+    if (coding != SMAX_HIP_COLUMNS_32) M.rel = A->col_rel;
+    if (coding == SMAX_HIP_COLUMNS_AUTO && A->col_idx && in.n_rows_padded % kWave == 0) {
+        M.idx = A->col_idx;
+        M.idx_row = A->col_idx_row;
+        M.tab = A->col_tab;
+    }
+    return M;
 }
 
 }  // namespace smaxhip
@@ -853,7 +1024,7 @@ int smax_hip_cg_f64(smax_hip_ctx ctx, smax_hip_scs A, const double *b,
     rc = prepare(ctx, A, -1, false, &rep->setup_seconds);
     if (rc) return rc;
     const smax_hip_scs_info &in = A->info;
-    const Mat M = slot_matrix(A);
+    const Mat M = slot_matrix(A, prm->column_coding);
     const double *val = static_cast<const double *>(in.val);
     const int64_t n = M.n, batch = prm->check_every > 0 ? prm->check_every : 25;
     Work w(ctx);
@@ -919,7 +1090,8 @@ int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
     rc = prepare(ctx, A, lo, diag, &rep->setup_seconds);
     if (rc) return rc;
     const smax_hip_scs_info &in = A->info;
-    const Mat M = slot_matrix(A);
+    const Mat M = slot_matrix(A, prm->column_coding);
+
     const double *val = static_cast<const double *>(in.val);
     const int64_t n = M.n, batch = prm->check_every > 0 ? prm->check_every : 25;
     const int64_t max_inner = prm->max_inner > 0 ? prm->max_inner : prm->max_iters;

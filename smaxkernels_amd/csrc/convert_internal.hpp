@@ -30,6 +30,19 @@ struct smax_hip_scs_s {
     // (0 for padding).  col_rel_state: 0 not tried, 1 built, -1 does not fit.
     int16_t *col_rel = nullptr;
     int col_rel_state = 0;
+    // Built after col_rel when C = 64 and every chunk's elements use at most
+    // 32 distinct offsets: per chunk a table of its offsets (32 int16 entries)
+    // and per element a one-byte index into its chunk's table.  The indices
+    // of elements j..j+3 (j a multiple of 4) of one slot share a 32-bit word:
+    // chunk c's words start at word row col_idx_row[c] (rows of 64 words, one
+    // per slot) and element (j, lane) is byte j % 4 of word (col_idx_row[c] +
+    // j / 4) * 64 + lane.  col_tab_state: 0 not tried, 1 built, -1 does not
+    // fit.
+    uint32_t *col_idx = nullptr;
+    int64_t *col_idx_row = nullptr;
+    int64_t col_idx_rows = 0;     // word rows in all
+    int16_t *col_tab = nullptr;
+    int col_tab_state = 0;
     smax_hip_scs_sched sched;     // built with the conversion
     // Built on first use by the SDDMM: row of every stored element (-1 for
     // padding), so an element's row needs no search.

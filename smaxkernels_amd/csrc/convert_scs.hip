@@ -722,6 +722,9 @@ int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs) {
     scs_free_arrays(scs->info);
     dev_free(scs->col_slot);
     dev_free(scs->col_rel);
+    dev_free(scs->col_idx);
+    dev_free(scs->col_idx_row);
+    dev_free(scs->col_tab);
     dev_free(scs->val_f32);
     dev_free(scs->val_bf16);
     dev_free(scs->diag_f32);
@@ -737,6 +740,16 @@ int smax_hip_scs_refresh_values(smax_hip_ctx ctx, smax_hip_scs scs,
     SMAX_HIP_REQUIRE(ctx, scs && crs_val, "smax_hip_scs_refresh_values: null");
     const smax_hip_scs_info &in = scs->info;
     const int64_t m = in.n_elements;
+    // the solvers' rounded copies follow the old values: drop them (rebuilt
+    // on the next solve)
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void **p : {reinterpret_cast<void **>(&scs->val_f32),
+                     reinterpret_cast<void **>(&scs->val_bf16),
+                     reinterpret_cast<void **>(&scs->diag_f32)}) {
+        dev_free(*p);
+        *p = nullptr;
+    }
+    scs->bf16_split = -1;
     if (m == 0) return SMAX_HIP_SUCCESS;
     switch (in.val_bytes) {
     case 8:

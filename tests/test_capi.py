@@ -38,7 +38,7 @@ def test_binding_covers_the_header():
 
 def test_abi_version_and_type_checks():
     lib = capi.load()
-    assert lib.smax_hip_abi_version() == 1
+    assert lib.smax_hip_abi_version() == 2
     assert lib.smax_hip_check_types(capi.INT32, capi.F64) == capi.SUCCESS
     assert lib.smax_hip_check_types(capi.INT32, capi.BF16) == capi.SUCCESS
     for it in (capi.INT16, capi.INT64, capi.UINT16, capi.UINT32, capi.UINT64):

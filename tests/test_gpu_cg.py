@@ -17,7 +17,7 @@ def true_rel_residual(A, b, x):
 
 
 def solve(ctx, A, C_, sigma, mixed, lo=capi.BF16, diag=True, tol=1e-10,
-          inner=1e-4, max_iters=20000, restart=0):
+          inner=1e-4, max_iters=20000, restart=0, coding=0):
     rp, col, v = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
     S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, 8, C_, sigma)
     b = matgen.uniform_f64(matgen.SEED_B, A.n_rows)
@@ -25,7 +25,7 @@ def solve(ctx, A, C_, sigma, mixed, lo=capi.BF16, diag=True, tol=1e-10,
     prm = capi.CgParams(rel_tol=tol, max_iters=max_iters, check_every=10,
                         inner_rel_tol=inner, max_inner=5000,
                         inner_value_type=lo, keep_diag_f32=int(diag),
-                        restart_inner=restart)
+                        restart_inner=restart, column_coding=coding)
     rep = ctx.cg(S, db, dx, prm, mixed)
     x = ctx.download(dx)
     S.destroy()
@@ -57,6 +57,54 @@ def test_cg_with_and_without_16bit_columns(ctx, sigma):
         rep, b, x = solve(ctx, A, 64, sigma, mixed=mixed, inner=1e-1)
         assert rep.status == capi.SUCCESS
         assert true_rel_residual(A, b, x) <= 1e-10
+
+
+@pytest.mark.parametrize("sigma", [1, 64])
+def test_column_codings_give_identical_iterates(ctx, sigma):
+    """One-byte table indices (a 27-point stencil chunk uses at most 27 slot
+    offsets at sigma = 1), 16-bit offsets and 32-bit columns name the same
+    column for every element: FP64 CG and BF16-refined CG give bit-identical
+    solutions, iteration counts and residuals with each.  At sigma = 64 rows
+    are permuted inside chunks, which may leave a chunk with too many
+    distinct offsets for a table; the automatic choice then falls back."""
+    A = matgen.stencil27(40)
+    for mixed in (False, True):
+        runs = [solve(ctx, A, 64, sigma, mixed=mixed, inner=1e-1, coding=c)
+                for c in (0, 2, 1)]
+        for rep, _, x in runs:
+            assert rep.status == capi.SUCCESS
+        rep0, b, x0 = runs[0]
+        assert true_rel_residual(A, b, x0) <= 1e-10
+        for rep, _, x in runs[1:]:
+            assert rep.iterations == rep0.iterations
+            assert rep.outer_iterations == rep0.outer_iterations
+            assert rep.rel_residual == rep0.rel_residual
+            assert np.array_equal(x.view(np.uint64), x0.view(np.uint64))
+
+
+def test_solvers_follow_refreshed_values(ctx):
+    """The solvers keep rounded (FP32/BF16) copies of the values; refreshing
+    the handle's values drops them, so a second solve runs on the new
+    matrix (2A here: same pattern, still SPD)."""
+    A = matgen.stencil27(24)
+    rp, col, v = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
+    S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, 8, 64, 1)
+    b = matgen.uniform_f64(matgen.SEED_B, A.n_rows)
+    db = ctx.upload(b)
+    prm = capi.CgParams(rel_tol=1e-10, max_iters=20000, check_every=10,
+                        inner_rel_tol=1e-1, max_inner=5000,
+                        inner_value_type=capi.BF16, keep_diag_f32=1)
+    for scale in (1.0, 2.0):
+        if scale != 1.0:
+            S.refresh_values(ctx.upload(A.val * scale))
+        for mixed in (True, False):
+            dx = ctx.zeros(A.n_rows, np.float64)
+            rep = ctx.cg(S, db, dx, prm, mixed)
+            assert rep.status == capi.SUCCESS
+            x = ctx.download(dx)
+            r = b - scale * oracle.crs_spmv(A.row_ptr, A.col, A.val, x)
+            assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.5e-10
+    S.destroy()
 
 
 @pytest.mark.parametrize("restart", [0, 1])
