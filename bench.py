@@ -209,7 +209,10 @@ def cg_section(ctx) -> dict:
                 "iterations": rep.iterations, "outer": rep.outer_iterations,
                 "rel_residual_gpu": rep.rel_residual,
                 "rel_residual_host_fp64": float(np.linalg.norm(r) / np.linalg.norm(b)),
-                "setup_seconds": round(rep.setup_seconds, 5)}, x
+                "setup_seconds": round(rep.setup_seconds, 5),
+                "column_coding": {capi.COLUMNS_8: "8-bit table indices",
+                                  capi.COLUMNS_16: "16-bit offsets",
+                                  capi.COLUMNS_32: "32-bit columns"}.get(rep.column_coding)}, x
 
     fp64, x64 = solve(False)
     bf16, xbf = solve(True, capi.BF16)

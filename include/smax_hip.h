@@ -408,7 +408,9 @@ typedef enum {
                                   else 16-bit offsets from the slot, else
                                   32-bit columns                            */
     SMAX_HIP_COLUMNS_32 = 1,   /* 32-bit slot-space columns                  */
-    SMAX_HIP_COLUMNS_16 = 2    /* 16-bit offsets when they fit, else 32-bit  */
+    SMAX_HIP_COLUMNS_16 = 2,   /* 16-bit offsets when they fit, else 32-bit  */
+    SMAX_HIP_COLUMNS_8 = 3     /* one-byte table indices: as a request, the
+                                  same as AUTO; reported when they ran      */
 } smax_hip_column_coding;
 
 typedef struct {
@@ -435,6 +437,8 @@ typedef struct {
     double rel_residual;     /* final ||b - A x||_2/||b||_2, FP64, on GPU   */
     double seconds;          /* wall time of the solve (setup excluded)     */
     double setup_seconds;    /* one-time build of the solver's matrix copies */
+    int32_t column_coding;   /* the coding the products used: 1, 2 or 3
+                                (SMAX_HIP_COLUMNS_32 / _16 / _8)             */
 } smax_hip_cg_report;
 
 /*

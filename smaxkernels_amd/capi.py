@@ -24,6 +24,7 @@ ERROR_NOT_CONVERGED = 8
 
 F64, F32, BF16 = 0, 1, 2
 INT16, INT32, INT64, UINT16, UINT32, UINT64 = range(6)
+COLUMNS_AUTO, COLUMNS_32, COLUMNS_16, COLUMNS_8 = range(4)
 
 VALUE_NP = {F64: np.float64, F32: np.float32, BF16: np.uint16}
 VECTOR_NP = {F64: np.float64, F32: np.float32, BF16: np.float32}
@@ -65,7 +66,7 @@ class CgParams(C.Structure):
 class CgReport(C.Structure):
     _fields_ = [("iterations", i64), ("outer_iterations", i64),
                 ("rel_residual", C.c_double), ("seconds", C.c_double),
-                ("setup_seconds", C.c_double)]
+                ("setup_seconds", C.c_double), ("column_coding", C.c_int32)]
 
 
 _SIGS = {

@@ -574,6 +574,9 @@ struct Mat {
     const int64_t *idx_row;
     const int16_t *tab;
     int mode() const { return idx ? kTab8 : rel ? kRel16 : kCol32; }
+    int32_t coding() const {
+        return idx ? SMAX_HIP_COLUMNS_8 : rel ? SMAX_HIP_COLUMNS_16 : SMAX_HIP_COLUMNS_32;
+    }
 };
 
 // q = A p with partial dots, through the matrix's column coding.
@@ -1000,7 +1003,8 @@ static Mat slot_matrix(smax_hip_scs A, int32_t coding) {
     Mat M{in.n_rows_padded, in.C, in.chunk_ptr, in.chunk_lengths,
           A->col_slot ? A->col_slot : in.col, nullptr, nullptr, nullptr, nullptr};
     if (coding != SMAX_HIP_COLUMNS_32) M.rel = A->col_rel;
-    if (coding == SMAX_HIP_COLUMNS_AUTO && A->col_idx && in.n_rows_padded % kWave == 0) {
+    if ((coding == SMAX_HIP_COLUMNS_AUTO || coding == SMAX_HIP_COLUMNS_8) && A->col_idx &&
+        in.n_rows_padded % kWave == 0) {
         M.idx = A->col_idx;
         M.idx_row = A->col_idx_row;
         M.tab = A->col_tab;
@@ -1025,6 +1029,7 @@ int smax_hip_cg_f64(smax_hip_ctx ctx, smax_hip_scs A, const double *b,
     if (rc) return rc;
     const smax_hip_scs_info &in = A->info;
     const Mat M = slot_matrix(A, prm->column_coding);
+    rep->column_coding = M.coding();
     const double *val = static_cast<const double *>(in.val);
     const int64_t n = M.n, batch = prm->check_every > 0 ? prm->check_every : 25;
     Work w(ctx);
@@ -1091,6 +1096,7 @@ int smax_hip_ircg(smax_hip_ctx ctx, smax_hip_scs A, const double *b, double *x,
     if (rc) return rc;
     const smax_hip_scs_info &in = A->info;
     const Mat M = slot_matrix(A, prm->column_coding);
+    rep->column_coding = M.coding();
 
     const double *val = static_cast<const double *>(in.val);
     const int64_t n = M.n, batch = prm->check_every > 0 ? prm->check_every : 25;

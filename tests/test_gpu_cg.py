@@ -70,9 +70,14 @@ def test_column_codings_give_identical_iterates(ctx, sigma):
     A = matgen.stencil27(40)
     for mixed in (False, True):
         runs = [solve(ctx, A, 64, sigma, mixed=mixed, inner=1e-1, coding=c)
-                for c in (0, 2, 1)]
+                for c in (capi.COLUMNS_AUTO, capi.COLUMNS_16, capi.COLUMNS_32)]
         for rep, _, x in runs:
             assert rep.status == capi.SUCCESS
+        # the coding that ran: tables at sigma = 1, 16-bit and 32-bit as asked
+        used = [rep.column_coding for rep, _, _ in runs]
+        assert used[1:] == [capi.COLUMNS_16, capi.COLUMNS_32]
+        assert used[0] == capi.COLUMNS_8 if sigma == 1 else used[0] in (capi.COLUMNS_8,
+                                                                        capi.COLUMNS_16)
         rep0, b, x0 = runs[0]
         assert true_rel_residual(A, b, x0) <= 1e-10
         for rep, _, x in runs[1:]:
