@@ -285,7 +285,9 @@ int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                        const void *x, void *y, int unpermuted);
 
 /* Re-gathers the SELL values from a CRS value array of the same pattern
- * (val_sell[e] = crs_val[src_index[e]], 0 for padding). */
+ * (val_sell[e] = crs_val[src_index[e]], 0 for padding).  Asynchronous, except
+ * when a solver left rounded value copies on the handle: those are dropped
+ * (after a stream synchronisation) and rebuilt by the next solve. */
 int smax_hip_scs_refresh_values(smax_hip_ctx ctx, smax_hip_scs scs,
                                 const void *crs_val);
 

@@ -741,15 +741,17 @@ int smax_hip_scs_refresh_values(smax_hip_ctx ctx, smax_hip_scs scs,
     const smax_hip_scs_info &in = scs->info;
     const int64_t m = in.n_elements;
     // the solvers' rounded copies follow the old values: drop them (rebuilt
-    // on the next solve)
-    (void)hipStreamSynchronize(ctx->stream);
-    for (void **p : {reinterpret_cast<void **>(&scs->val_f32),
-                     reinterpret_cast<void **>(&scs->val_bf16),
-                     reinterpret_cast<void **>(&scs->diag_f32)}) {
-        dev_free(*p);
-        *p = nullptr;
+    // on the next solve); without copies the refresh stays asynchronous
+    if (scs->val_f32 || scs->val_bf16 || scs->diag_f32) {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (void **p : {reinterpret_cast<void **>(&scs->val_f32),
+                         reinterpret_cast<void **>(&scs->val_bf16),
+                         reinterpret_cast<void **>(&scs->diag_f32)}) {
+            dev_free(*p);
+            *p = nullptr;
+        }
+        scs->bf16_split = -1;
     }
-    scs->bf16_split = -1;
     if (m == 0) return SMAX_HIP_SUCCESS;
     switch (in.val_bytes) {
     case 8:
