@@ -87,6 +87,43 @@ def test_column_codings_give_identical_iterates(ctx, sigma):
             assert np.array_equal(x.view(np.uint64), x0.view(np.uint64))
 
 
+def random_band_spd(n, half, per_row, seed):
+    """Symmetric, strictly diagonally dominant (so SPD) matrix with per_row
+    random off-diagonal pairs per row inside +-half of the diagonal."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    i = np.repeat(np.arange(n), per_row)
+    j = i + rng.integers(-half, half + 1, i.size)
+    keep = (i != j) & (j >= 0) & (j < n)
+    j = np.where(keep, j, 0)
+    v = -rng.uniform(0.1, 1.0, keep.sum())
+    M = sp.coo_matrix((v, (i[keep], j[keep])), shape=(n, n))
+    M = (M + M.T).tocsr()
+    M.sum_duplicates()
+    M = M + sp.diags(np.asarray(abs(M).sum(axis=1)).ravel() + 1.0)
+    M = M.tocsr()
+    M.sort_indices()
+    return matgen.CRS(n, n, M.indptr.astype(np.int32), M.indices.astype(np.int32),
+                      M.data.astype(np.float64))
+
+
+def test_column_tables_fall_back_when_chunks_use_many_offsets(ctx):
+    """Random band rows: a 64-row chunk uses far more than 32 distinct slot
+    offsets (all within int16), so the solvers read 16-bit offsets; forcing
+    32-bit columns gives the same iterates."""
+    A = random_band_spd(20000, 1000, 8, 5)
+    for mixed in (False, True):
+        runs = [solve(ctx, A, 64, 1, mixed=mixed, inner=1e-1, coding=c)
+                for c in (capi.COLUMNS_AUTO, capi.COLUMNS_32)]
+        assert [rep.column_coding for rep, _, _ in runs] == [capi.COLUMNS_16,
+                                                              capi.COLUMNS_32]
+        (rep0, b, x0), (rep1, _, x1) = runs
+        assert rep0.status == capi.SUCCESS and rep1.status == capi.SUCCESS
+        assert rep0.iterations == rep1.iterations
+        assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64))
+        assert true_rel_residual(A, b, x0) <= 1e-10
+
+
 def test_solvers_follow_refreshed_values(ctx):
     """The solvers keep rounded (FP32/BF16) copies of the values; refreshing
     the handle's values drops them, so a second solve runs on the new
