@@ -157,7 +157,12 @@ int smax_hip_crs_plan_destroy(smax_hip_ctx ctx, smax_hip_crs_plan plan);
  * chain in CRS order (bit-identical to the reference CPU kernel).  A plan of
  * the same pattern (smax_hip_crs_plan_create; may be NULL) makes row-major
  * products run long rows first and the rest by length inside 65536-row
- * windows; results are the same with or without it.
+ * windows; results are the same with or without it.  Column-major products
+ * with k >= 2 copy X to row-major, run the row-major kernel (with the plan)
+ * and copy Y back, in a workspace the context allocates on first use and
+ * grows when a larger product needs it (those calls synchronise the stream;
+ * the others only launch kernels).  X must hold (k-1)*ldx + n_cols entries
+ * (row-major: (n_cols-1)*ldx + k).
  */
 int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                       int64_t n_cols, int64_t nnz, const int32_t *row_ptr,

@@ -19,6 +19,10 @@ struct smax_hip_ctx_s {
     std::string last_error;
     // Small pinned host word used to read back device counters.
     int64_t *host_word = nullptr;
+    // Device workspace of entry points that stage operands (column-major
+    // CRS SpMM), grown on demand and kept until the context is destroyed.
+    void *work = nullptr;
+    size_t work_bytes = 0;
 };
 
 namespace smaxhip {
@@ -38,6 +42,10 @@ int check_launch(smax_hip_ctx ctx, const char *kernel);
 // Allocation helpers that record failures.
 int dev_alloc(smax_hip_ctx ctx, void **p, size_t bytes, const char *what);
 void dev_free(void *p);
+
+// ctx's workspace of at least `bytes` (256-byte aligned).  Growing it waits
+// for the stream (queued work may still use the old buffer) and reallocates.
+int ctx_workspace(smax_hip_ctx ctx, size_t bytes, void **out);
 
 // Device counter read-back (int64 at a device address), synchronous.
 int read_i64(smax_hip_ctx ctx, const int64_t *dptr, int64_t *out);

@@ -67,6 +67,20 @@ void dev_free(void *p) {
     if (p) (void)hipFree(p);
 }
 
+int ctx_workspace(smax_hip_ctx ctx, size_t bytes, void **out) {
+    if (ctx->work_bytes < bytes) {
+        SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->work) (void)hipFree(ctx->work);
+        ctx->work = nullptr;
+        ctx->work_bytes = 0;
+        const int rc = dev_alloc(ctx, &ctx->work, bytes, "workspace");
+        if (rc) return rc;
+        ctx->work_bytes = bytes;
+    }
+    *out = ctx->work;
+    return SMAX_HIP_SUCCESS;
+}
+
 int read_i64(smax_hip_ctx ctx, const int64_t *dptr, int64_t *out) {
     SMAX_HIP_CALL(ctx, hipMemcpyAsync(ctx->host_word, dptr, sizeof(int64_t),
                                       hipMemcpyDeviceToHost, ctx->stream));
@@ -226,6 +240,7 @@ int smax_hip_ctx_destroy(smax_hip_ctx ctx) {
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     if (ctx->host_word) (void)hipHostFree(ctx->host_word);
+    if (ctx->work) (void)hipFree(ctx->work);
     delete ctx;
     return SMAX_HIP_SUCCESS;
 }

@@ -362,6 +362,45 @@ static int build_row_order(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
 
 using namespace smaxhip;
 
+namespace smaxhip {
+
+// Dense rows x k block between column-major (element (r, v) at r + v*ld) and
+// row-major (r*k + v) layouts, through a 32 x 33 LDS tile so both the read
+// and the write are coalesced.  COL_IN: column-major in, row-major out.
+template <typename T, bool COL_IN>
+__global__ __launch_bounds__(256) void k_relayout(int64_t rows, int64_t k,
+                                                  const T *__restrict__ in,
+                                                  int64_t ld, T *__restrict__ out) {
+    __shared__ T tile[32][33];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 32;
+    const int64_t v0 = static_cast<int64_t>(blockIdx.y) * 32;
+    const int tx = static_cast<int>(threadIdx.x) & 31, ty = static_cast<int>(threadIdx.x) >> 5;
+#pragma unroll
+    for (int j = 0; j < 32; j += 8) {
+        // the read walks the unit-stride index of `in` with tx
+        const int64_t r = COL_IN ? r0 + tx : r0 + ty + j;
+        const int64_t v = COL_IN ? v0 + ty + j : v0 + tx;
+        if (r < rows && v < k) {
+            const T e = COL_IN ? in[r + v * ld] : in[r * k + v];
+            if (COL_IN) tile[ty + j][tx] = e;   // tile[v - v0][r - r0]
+            else tile[tx][ty + j] = e;          // tile[v - v0][r - r0]
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 32; j += 8) {
+        // the write walks the unit-stride index of `out` with tx
+        const int64_t r = COL_IN ? r0 + ty + j : r0 + tx;
+        const int64_t v = COL_IN ? v0 + tx : v0 + ty + j;
+        if (r < rows && v < k) {
+            if (COL_IN) out[r * k + v] = tile[tx][ty + j];
+            else out[r + v * ld] = tile[ty + j][tx];
+        }
+    }
+}
+
+}  // namespace smaxhip
+
 extern "C" {
 
 int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
@@ -511,6 +550,38 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
     if (plan)
         SMAX_HIP_REQUIRE(ctx, plan->n_rows == n_rows && plan->nnz == nnz,
                          "smax_hip_crs_spmm: plan built for another pattern");
+    if (!row_major && k >= 2) {
+        // Column-major operands run through row-major copies in the context's
+        // workspace: each X row (the k values a nonzero multiplies) becomes
+        // contiguous, so the row-major kernels and the plan's row order apply.
+        // Every output is the same chain, so Y is unchanged bit for bit.
+        SMAX_HIP_DISPATCH_VT(value_type, {
+            using A = typename vt<VT>::vec_t;
+            const size_t xb = (static_cast<size_t>(n_cols) * k * sizeof(A) + 255) / 256 * 256;
+            const size_t yb = static_cast<size_t>(n_rows) * k * sizeof(A);
+            void *w = nullptr;
+            int rc = ctx_workspace(ctx, xb + yb, &w);
+            if (rc) return rc;
+            A *Xr = static_cast<A *>(w);
+            A *Yr = reinterpret_cast<A *>(static_cast<char *>(w) + xb);
+            const unsigned gy = static_cast<unsigned>(ceil_div(k, 32));
+            hipLaunchKernelGGL((k_relayout<A, true>),
+                               dim3(static_cast<unsigned>(ceil_div(n_cols, 32)), gy),
+                               dim3(256), 0, ctx->stream, n_cols, k,
+                               static_cast<const A *>(X), ldx, Xr);
+            SMAX_HIP_LAUNCHED(ctx, "k_relayout");
+            rc = smax_hip_crs_spmm(ctx, value_type, n_rows, n_cols, nnz, row_ptr, col, val,
+                                   k, 1, Xr, k, Yr, k, plan);
+            if (rc) return rc;
+            hipLaunchKernelGGL((k_relayout<A, false>),
+                               dim3(static_cast<unsigned>(ceil_div(n_rows, 32)), gy),
+                               dim3(256), 0, ctx->stream, n_rows, k, Yr, ldy,
+                               static_cast<A *>(Y));
+            SMAX_HIP_LAUNCHED(ctx, "k_relayout");
+            return SMAX_HIP_SUCCESS;
+        });
+        return SMAX_HIP_ERROR_UNSUPPORTED;
+    }
     const int32_t *order = plan ? plan->row_order : nullptr;
     const int64_t spmm_lds = plan ? plan->spmm_lds : 0;
     const int64_t threads = n_rows * k;

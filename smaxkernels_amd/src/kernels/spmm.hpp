@@ -83,7 +83,7 @@ class SpMMKernel : public Kernel {
         rc |= val.upload(c, A.val, vb * A.nnz);
         rc |= X.upload(c, args.X.val, xb * args.X.n_rows * args.X.n_cols);
         rc |= Y.upload(c, args.Y.val, xb * args.Y.n_rows * args.Y.n_cols);
-        if (!rc && flags.vec_row_major &&
+        if (!rc &&
             (!plan || plan_row_ptr != A.row_ptr || plan_rows != A.n_rows || plan_nnz != A.nnz)) {
             if (plan) smax_hip_crs_plan_destroy(c, plan);
             plan = nullptr;
@@ -108,13 +108,23 @@ class SpMMKernel : public Kernel {
         // reference does (spmm_cpu_crs_impl.hpp:29,41)
         const long long ld = flags.vec_row_major ? static_cast<long long>(k)
                                                  : static_cast<long long>(A.n_rows);
+        // the last element either operand's addressing reaches must lie in the
+        // registered buffer (column-major with n_cols != n_rows can overrun)
+        const ULL x_need = flags.vec_row_major ? A.n_cols * k : (k - 1) * ld + A.n_cols;
+        const ULL y_need = flags.vec_row_major ? A.n_rows * k : (k - 1) * ld + A.n_rows;
+        if (k == 0 || X_offset + x_need > args.X.n_rows * args.X.n_cols ||
+            Y_offset + y_need > args.Y.n_rows * args.Y.n_cols) {
+            timers->get("apply")->stop();
+            ErrorHandler::warning("SpMM: X or Y is smaller than the product addresses");
+            return 1;
+        }
         int st = smax_hip_crs_spmm(
             stream.get(), SPMV::HIP::hip_value_type(k_ctx->float_type), A.n_rows,
             A.n_cols, A.nnz, row_ptr.as<int32_t>(), col.as<int32_t>(), val.get(), k,
             flags.vec_row_major ? 1 : 0,
             static_cast<const char *>(X.get()) + xb * X_offset, ld,
             static_cast<char *>(Y.get()) + xb * Y_offset, ld,
-            flags.vec_row_major ? plan : nullptr);
+            plan);
         timers->get("apply")->stop();
         return PLATFORM::HIP::report(stream.get(), st, "SpMM launch");
     }

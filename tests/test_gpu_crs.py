@@ -128,6 +128,36 @@ def test_crs_spmm_bit_exact(ctx, vt, k, row_major, planned):
 
 
 @pytest.mark.parametrize("vt", VTS)
+def test_crs_spmm_column_major_padded_and_growing(ctx, vt):
+    """Column-major operands with leading dimensions past n_rows, widths off
+    the 32-wide staging tile, and a workspace that grows (k = 64 after 37)
+    and is then reused (k = 3): Y equals the reference column by column, and
+    the padding between columns is left untouched."""
+    A = matgen.lomax_band(9000, half=1000)
+    n = A.n_rows
+    rp, col = ctx.upload(A.row_ptr), ctx.upload(A.col)
+    for k in (37, 64, 3):
+        val, X = operands(vt, A, k)  # X: n x k column-major, ld n
+        v = ctx.upload(val)
+        ldx, ldy = n + 5, n + 3
+        Xp = np.zeros(ldx * k, X.dtype)
+        for c in range(k):
+            Xp[c * ldx:c * ldx + n] = X[c * n:(c + 1) * n]
+        dX = ctx.upload(Xp)
+        sentinel = np.full(ldy * k, 7.0, X.dtype)
+        dY = ctx.upload(sentinel)
+        ctx.crs_spmm(vt, n, n, A.nnz, rp, col, v, k, False, dX, ldx, dY, ldy)
+        got = ctx.download(dY).reshape(k, ldy)
+        want = oracle.crs_spmm(A.row_ptr, A.col, val, k, False, X).reshape(k, n)
+        assert_bitwise(got[:, :n], want, f"k={k}")
+        assert (got[:, n:] == 7.0).all()
+        for d in (v, dX, dY):
+            d.free()
+    for d in (rp, col):
+        d.free()
+
+
+@pytest.mark.parametrize("vt", VTS)
 def test_crs_spmm_row_major_unaligned_operands(ctx, vt):
     """k = 32 row-major with X and Y one entry past a 16-byte boundary takes
     the one-column-per-lane kernel instead of the 16-byte one; both run the
