@@ -381,6 +381,10 @@ def c4_section(ctx, steps, warmup) -> dict:
                                             val, k, True, dX, k, dY, k),
         "forward_crs_plan": lambda: ctx.crs_spmm(capi.F32, A.n_rows, A.n_cols, A.nnz, rp,
                                                  col, val, k, True, dX, k, dY, k, plan),
+        # the reference's default layout: column-major X and Y (ld = n_rows)
+        "forward_crs_colmajor_plan": lambda: ctx.crs_spmm(
+            capi.F32, A.n_rows, A.n_cols, A.nnz, rp, col, val, k, False, dX, A.n_rows, dY,
+            A.n_rows, plan),
         "forward_sell": lambda: S.apply_spmm(capi.F32, k, dX, dY),
         "adjoint_sell_AT": lambda: T.apply_spmm(capi.F32, k, dW, dG),
         "sddmm_sell": lambda: S.sddmm(capi.F32, k, dW, dX, ddv),
