@@ -207,6 +207,22 @@ def test_bcrs_conversion_long_rows_read_from_memory(ctx, vt, cm):
     b.destroy()
 
 
+def test_bcrs_conversion_of_a_matrix_without_entries(ctx):
+    """nnz = 0: no blocks, an all-zero block row pointer, and an SpMV that
+    writes zeros."""
+    rp = np.zeros(9, np.int32)
+    b = gpu_bcrs(ctx, 8, 8, rp, np.zeros(0, np.int32), np.zeros(0), 2, 2, 2, 2, False)
+    i = b.info
+    assert i.n_blocks == 0 and i.n_block_rows == 4
+    assert (b.arrays(np.float64)["row_ptr"] == 0).all()
+    dx, dy = ctx.upload(np.ones(8)), ctx.upload(np.full(8, 5.0))
+    ctx.bcrs_spmv(capi.F64, 4, 2, 2, 2, 2, i.col, i.row_ptr, i.val, False, dx, dy)
+    assert (ctx.download(dy) == 0).all()
+    for d in (dx, dy):
+        d.free()
+    b.destroy()
+
+
 def test_bcrs_conversion_rejects_columns_out_of_range(ctx):
     """A column index outside [0, n_cols) would place a value outside its
     block: the conversion refuses the matrix."""
