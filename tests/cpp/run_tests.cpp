@@ -252,6 +252,22 @@ TEST(gpu, spmm_kats) {  // spmm_test.hpp (column-major), spmm_rowmaj_test.hpp
     }
 }
 
+TEST(gpu, spmm_column_major_refuses_overrun) {
+    // 5 x 3 matrix, column-major with the reference's leading dimension
+    // n_rows = 5: X (3 x k) would be read up to (k-1)*5 + 3 > 3k entries, so
+    // run() reports failure instead of reading past the registered buffer
+    SMAX::Interface smax;
+    const int k = 2;
+    std::vector<int> rp = {0, 1, 2, 3, 3, 3}, col = {0, 1, 2};
+    std::vector<double> val = {1.0, 2.0, 3.0}, X(3 * k, 1.0), Y(5 * k, 0.0);
+    smax.register_kernel("spmm_overrun", KernelType::SPMM, PlatformType::HIP);
+    auto *kk = smax.kernel("spmm_overrun");
+    kk->register_A(5, 3, 3, col.data(), rp.data(), val.data());
+    kk->register_B(3, k, X.data());
+    kk->register_C(5, k, Y.data());
+    check(kk->run() != 0, "overrun refused");
+}
+
 TEST(gpu, float32_and_bfloat16_values) {
     // 2-D 5-point Laplacian on 32^2 with small-integer x: exact in every type
     const int n = 32, N = n * n;
