@@ -160,9 +160,15 @@ int smax_hip_crs_plan_destroy(smax_hip_ctx ctx, smax_hip_crs_plan plan);
  * windows; results are the same with or without it.  Column-major products
  * with k >= 2 copy X to row-major, run the row-major kernel (with the plan)
  * and copy Y back, in a workspace the context allocates on first use and
- * grows when a larger product needs it (those calls synchronise the stream;
- * the others only launch kernels).  X must hold (k-1)*ldx + n_cols entries
- * (row-major: (n_cols-1)*ldx + k).
+ * grows when a larger product needs it (those calls allocate; the others only
+ * launch kernels).  An outgrown workspace is kept until the context is
+ * destroyed, so work already queued and HIP graphs captured before the growth
+ * stay valid; calls that overlap in time on different streams of one context
+ * share the workspace and race.  X must hold (k-1)*ldx + n_cols entries
+ * (row-major: (n_cols-1)*ldx + k).  Leading dimensions: row-major ldx, ldy
+ * >= k; column-major ldx >= min(n_rows, n_cols) (columns of X may overlap,
+ * as with the reference's ldx = n_rows when n_cols > n_rows) and, for
+ * k >= 2, ldy >= n_rows (columns of Y must not overlap).
  */
 int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                       int64_t n_cols, int64_t nnz, const int32_t *row_ptr,
@@ -486,6 +492,11 @@ int smax_hip_stream_copy(smax_hip_ctx ctx, const double *src, double *dst,
                          int64_t n);
 int smax_hip_stream_triad(smax_hip_ctx ctx, double *a, const double *b,
                           const double *c, double s, int64_t n);
+/* The copy with loads_per_lane (1, 2, 4 or 8) independent 16-byte loads in
+ * flight per lane before its stores: a second method for the stream peak
+ * (src and dst 16-byte aligned).  No reference counterpart. */
+int smax_hip_stream_copy_wide(smax_hip_ctx ctx, const double *src, double *dst,
+                              int64_t n, int loads_per_lane);
 /* Random-gather reference for SpMVs whose x accesses are scattered: lane i
  * (i < n) reads per_lane (8, 16 or 32) entries of x[n] at pseudo-random
  * positions of a window of `window` entries centred on i (shifted to stay

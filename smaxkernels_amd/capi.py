@@ -128,6 +128,7 @@ _SIGS = {
                        C.POINTER(CgReport)], C.c_int),
     "smax_hip_stream_copy": ([vp, vp, vp, i64], C.c_int),
     "smax_hip_stream_triad": ([vp, vp, vp, vp, C.c_double, i64], C.c_int),
+    "smax_hip_stream_copy_wide": ([vp, vp, vp, i64, C.c_int], C.c_int),
     "smax_hip_gather_rate": ([vp, vp, vp, i64, i64, C.c_int], C.c_int),
 }
 
@@ -365,6 +366,10 @@ class Context:
     def stream_copy(self, src, dst, n) -> None:
         self._check(self.lib.smax_hip_stream_copy(self.handle, _ptr(src),
                                                   _ptr(dst), n))
+
+    def stream_copy_wide(self, src, dst, n, loads_per_lane) -> None:
+        self._check(self.lib.smax_hip_stream_copy_wide(self.handle, _ptr(src), _ptr(dst),
+                                                       n, loads_per_lane))
 
     def stream_triad(self, a, b, c, s, n) -> None:
         self._check(self.lib.smax_hip_stream_triad(self.handle, _ptr(a),

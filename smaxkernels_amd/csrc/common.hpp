@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "smax_hip.h"
 
@@ -20,9 +21,12 @@ struct smax_hip_ctx_s {
     // Small pinned host word used to read back device counters.
     int64_t *host_word = nullptr;
     // Device workspace of entry points that stage operands (column-major
-    // CRS SpMM), grown on demand and kept until the context is destroyed.
+    // CRS SpMM), grown on demand.  A buffer that is outgrown is retired, not
+    // freed: work already queued, and HIP graphs captured with it, keep a
+    // valid address until the context is destroyed.
     void *work = nullptr;
     size_t work_bytes = 0;
+    std::vector<void *> retired_work;
 };
 
 namespace smaxhip {
@@ -43,8 +47,8 @@ int check_launch(smax_hip_ctx ctx, const char *kernel);
 int dev_alloc(smax_hip_ctx ctx, void **p, size_t bytes, const char *what);
 void dev_free(void *p);
 
-// ctx's workspace of at least `bytes` (256-byte aligned).  Growing it waits
-// for the stream (queued work may still use the old buffer) and reallocates.
+// ctx's workspace of at least `bytes` (256-byte aligned).  Growing it
+// allocates a new buffer and retires the old one (freed with the context).
 int ctx_workspace(smax_hip_ctx ctx, size_t bytes, void **out);
 
 // Device counter read-back (int64 at a device address), synchronous.

@@ -69,12 +69,11 @@ void dev_free(void *p) {
 
 int ctx_workspace(smax_hip_ctx ctx, size_t bytes, void **out) {
     if (ctx->work_bytes < bytes) {
-        SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->work) (void)hipFree(ctx->work);
-        ctx->work = nullptr;
-        ctx->work_bytes = 0;
-        const int rc = dev_alloc(ctx, &ctx->work, bytes, "workspace");
+        void *grown = nullptr;
+        const int rc = dev_alloc(ctx, &grown, bytes, "workspace");
         if (rc) return rc;
+        if (ctx->work) ctx->retired_work.push_back(ctx->work);
+        ctx->work = grown;
         ctx->work_bytes = bytes;
     }
     *out = ctx->work;
@@ -114,6 +113,33 @@ __global__ void copy_kernel(const double *__restrict__ src,
     for (int64_t k = i; k < n2; k += stride)
         __builtin_nontemporal_store(__builtin_nontemporal_load(s2 + k), d2 + k);
     if (i == 0 && (n & 1)) dst[n - 1] = src[n - 1];
+}
+
+// The copy with L independent 16-byte loads in flight per lane: a workgroup
+// iteration covers 256*L consecutive pairs, lane t takes pairs t + 256*l, so
+// every load instruction of a wavefront is one contiguous 1 KiB run.
+template <int L>
+__global__ __launch_bounds__(256) void copy_wide_kernel(const double *__restrict__ src,
+                                                        double *__restrict__ dst, int64_t n) {
+    const int64_t n2 = n / 2;
+    const f64x2 *s2 = reinterpret_cast<const f64x2 *>(src);
+    f64x2 *d2 = reinterpret_cast<f64x2 *>(dst);
+    const int64_t span = static_cast<int64_t>(256) * L;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * span; base < n2;
+         base += static_cast<int64_t>(gridDim.x) * span) {
+        f64x2 v[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const int64_t k = base + threadIdx.x + 256 * l;
+            if (k < n2) v[l] = __builtin_nontemporal_load(s2 + k);
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const int64_t k = base + threadIdx.x + 256 * l;
+            if (k < n2) __builtin_nontemporal_store(v[l], d2 + k);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
 }
 
 __global__ void triad_kernel(double *__restrict__ a,
@@ -236,11 +262,14 @@ int smax_hip_ctx_destroy(smax_hip_ctx ctx) {
     if (!ctx) return SMAX_HIP_SUCCESS;
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
+    if (ctx->stream && ctx->stream != ctx->own_stream)
+        (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     if (ctx->host_word) (void)hipHostFree(ctx->host_word);
     if (ctx->work) (void)hipFree(ctx->work);
+    for (void *p : ctx->retired_work) (void)hipFree(p);
     delete ctx;
     return SMAX_HIP_SUCCESS;
 }
@@ -346,6 +375,26 @@ int smax_hip_stream_copy(smax_hip_ctx ctx, const double *src, double *dst,
     hipLaunchKernelGGL(copy_kernel, stream_grid(n / 2), dim3(256), 0,
                        ctx->stream, src, dst, n);
     SMAX_HIP_LAUNCHED(ctx, "copy_kernel");
+    return SMAX_HIP_SUCCESS;
+}
+
+int smax_hip_stream_copy_wide(smax_hip_ctx ctx, const double *src, double *dst,
+                              int64_t n, int loads_per_lane) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    SMAX_HIP_REQUIRE(ctx, src && dst && n > 0, "smax_hip_stream_copy_wide: bad args");
+    SMAX_HIP_REQUIRE(ctx, aligned16(src, dst),
+                     "smax_hip_stream_copy_wide: pointers must be 16-byte aligned");
+    const dim3 g = stream_grid(ceil_div(n / 2, loads_per_lane));
+    switch (loads_per_lane) {
+    case 1: hipLaunchKernelGGL(copy_wide_kernel<1>, g, dim3(256), 0, ctx->stream, src, dst, n); break;
+    case 2: hipLaunchKernelGGL(copy_wide_kernel<2>, g, dim3(256), 0, ctx->stream, src, dst, n); break;
+    case 4: hipLaunchKernelGGL(copy_wide_kernel<4>, g, dim3(256), 0, ctx->stream, src, dst, n); break;
+    case 8: hipLaunchKernelGGL(copy_wide_kernel<8>, g, dim3(256), 0, ctx->stream, src, dst, n); break;
+    default:
+        return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                    "smax_hip_stream_copy_wide: loads_per_lane must be 1, 2, 4 or 8");
+    }
+    SMAX_HIP_LAUNCHED(ctx, "copy_wide_kernel");
     return SMAX_HIP_SUCCESS;
 }
 

@@ -545,8 +545,9 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
         SMAX_HIP_REQUIRE(ctx, ldx >= k && ldy >= k,
                          "smax_hip_crs_spmm: row-major ld must be >= k");
     else
-        SMAX_HIP_REQUIRE(ctx, ldx >= n_cols || ldx >= n_rows,
-                         "smax_hip_crs_spmm: column-major ldx too small");
+        SMAX_HIP_REQUIRE(ctx, (ldx >= n_cols || ldx >= n_rows) && (k == 1 || ldy >= n_rows),
+                         "smax_hip_crs_spmm: column-major ld too small (need ldx >= "
+                         "min(n_rows, n_cols) and ldy >= n_rows)");
     if (plan)
         SMAX_HIP_REQUIRE(ctx, plan->n_rows == n_rows && plan->nnz == nnz,
                          "smax_hip_crs_spmm: plan built for another pattern");

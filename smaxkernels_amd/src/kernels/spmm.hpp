@@ -72,7 +72,7 @@ class SpMMKernel : public Kernel {
 
     int initialize(ULL, ULL, ULL) override {
         if (!hip()) return 1;
-        timers->get("initialize")->start();
+        ScopedPhase phase(timers.get(), "initialize");
         if (SPMV::HIP::check_types(k_ctx.get()) || stream.open()) return 1;
         smax_hip_ctx c = stream.get();
         const CRSMatrix &A = *args.A.crs;
@@ -94,13 +94,12 @@ class SpMMKernel : public Kernel {
             plan_rows = A.n_rows;
             plan_nnz = A.nnz;
         }
-        timers->get("initialize")->stop();
         return rc ? 1 : 0;
     }
 
     int apply(ULL, ULL X_offset, ULL Y_offset) override {
         if (!hip()) return 1;
-        timers->get("apply")->start();
+        ScopedPhase phase(timers.get(), "apply");
         const CRSMatrix &A = *args.A.crs;
         const ULL k = args.X.n_cols;
         const size_t xb = SPMV::HIP::vector_bytes(k_ctx->float_type);
@@ -114,8 +113,7 @@ class SpMMKernel : public Kernel {
         const ULL y_need = flags.vec_row_major ? A.n_rows * k : (k - 1) * ld + A.n_rows;
         if (k == 0 || X_offset + x_need > args.X.n_rows * args.X.n_cols ||
             Y_offset + y_need > args.Y.n_rows * args.Y.n_cols) {
-            timers->get("apply")->stop();
-            ErrorHandler::warning("SpMM: X or Y is smaller than the product addresses");
+                ErrorHandler::warning("SpMM: X or Y is smaller than the product addresses");
             return 1;
         }
         int st = smax_hip_crs_spmm(
@@ -125,19 +123,17 @@ class SpMMKernel : public Kernel {
             static_cast<const char *>(X.get()) + xb * X_offset, ld,
             static_cast<char *>(Y.get()) + xb * Y_offset, ld,
             plan);
-        timers->get("apply")->stop();
         return PLATFORM::HIP::report(stream.get(), st, "SpMM launch");
     }
 
     int finalize(ULL, ULL, ULL) override {
         if (!hip()) return 1;
-        timers->get("finalize")->start();
+        ScopedPhase phase(timers.get(), "finalize");
         int rc = PLATFORM::HIP::report(stream.get(), smax_hip_sync(stream.get()),
                                        "synchronize");
         if (!rc)
             rc = Y.download(args.Y.val, SPMV::HIP::vector_bytes(k_ctx->float_type) *
                                             args.Y.n_rows * args.Y.n_cols);
-        timers->get("finalize")->stop();
         return rc;
     }
 

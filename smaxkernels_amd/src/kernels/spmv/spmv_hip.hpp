@@ -62,7 +62,7 @@ inline int check_types(KernelContext *k) {
 
 inline int initialize(Timers *timers, KernelContext *k, Args *args, Flags *flags,
                       DeviceState &d) {
-    timers->get("initialize")->start();
+    ScopedPhase phase(timers, "initialize");
     if (check_types(k) || d.stream.open()) return 1;
     smax_hip_ctx ctx = d.stream.get();
     const size_t vb = value_bytes(k->float_type), xb = vector_bytes(k->float_type);
@@ -123,13 +123,12 @@ inline int initialize(Timers *timers, KernelContext *k, Args *args, Flags *flags
     y_size = std::max(y_size, args->y.n_rows);
     rc |= d.x.upload(ctx, args->x.val, xb * x_size);
     rc |= d.y.upload(ctx, args->y.val, xb * y_size);
-    timers->get("initialize")->stop();
     return rc ? 1 : 0;
 }
 
 inline int apply(Timers *timers, KernelContext *k, Args *args, Flags *flags,
                  DeviceState &d, ULL x_off, ULL y_off) {
-    timers->get("apply")->start();
+    ScopedPhase phase(timers, "apply");
     smax_hip_ctx ctx = d.stream.get();
     if (!ctx) {
         ErrorHandler::warning("[HIP] apply before initialize");
@@ -161,13 +160,12 @@ inline int apply(Timers *timers, KernelContext *k, Args *args, Flags *flags,
                                d.a0.as<int32_t>(), d.a1.as<int32_t>(), d.a2.get(),
                                x, y, d.plan);
     }
-    timers->get("apply")->stop();
     return report(ctx, st, "SpMV launch");
 }
 
 inline int finalize(Timers *timers, KernelContext *k, Args *args, Flags *flags,
                     DeviceState &d) {
-    timers->get("finalize")->start();
+    ScopedPhase phase(timers, "finalize");
     smax_hip_ctx ctx = d.stream.get();
     if (!ctx) {
         ErrorHandler::warning("[HIP] finalize before initialize");
@@ -182,7 +180,6 @@ inline int finalize(Timers *timers, KernelContext *k, Args *args, Flags *flags,
         y_size = args->A.crs->n_rows;
     int rc = report(ctx, smax_hip_sync(ctx), "synchronize");
     if (!rc) rc = d.y.download(args->y.val, vector_bytes(k->float_type) * y_size);
-    timers->get("finalize")->stop();
     return rc;
 }
 

@@ -35,4 +35,20 @@ class Timers {
     }
 };
 
+// Runs a phase's stopwatch for one scope: started on construction, stopped on
+// every way out of the scope (early error returns included).
+class ScopedPhase {
+    Stopwatch *sw_;
+
+  public:
+    ScopedPhase(Timers *timers, const char *phase) : sw_(timers->get(phase)) {
+        if (sw_) sw_->start();
+    }
+    ~ScopedPhase() {
+        if (sw_) sw_->stop();
+    }
+    ScopedPhase(const ScopedPhase &) = delete;
+    ScopedPhase &operator=(const ScopedPhase &) = delete;
+};
+
 } // namespace SMAX

@@ -189,3 +189,139 @@ def test_kat_spmm_on_gpu(ctx):
         ld = nv if row_major else 3
         ctx.crs_spmm(capi.F64, 3, 3, 5, rp, col, v, nv, row_major, dX, ld, dY, ld)
         assert almost_equal(ctx.download(dY), arr(k, "Y_expected", np.float64)).all()
+
+
+def rectangular(A, n_rows=None, n_cols=None):
+    """The first n_rows rows of A with the columns >= n_cols dropped."""
+    n_rows = A.n_rows if n_rows is None else n_rows
+    n_cols = A.n_cols if n_cols is None else n_cols
+    keep = A.col < n_cols
+    rows = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
+    keep &= rows < n_rows
+    lens = np.bincount(rows[keep], minlength=A.n_rows)[:n_rows]
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    return Csr(n_rows, n_cols, rp, A.col[keep].copy(), A.val[keep].copy())
+
+
+@pytest.mark.parametrize("vt", VTS)
+@pytest.mark.parametrize("shape", ["wide", "tall"])
+def test_crs_spmm_column_major_rectangular(ctx, vt, shape):
+    """Column-major products with n_cols != n_rows: X is staged over n_cols
+    rows and Y over n_rows, so a swap of the two extents would show here.
+    Padded leading dimensions (ldx >= n_cols, ldy > n_rows) leave the padding
+    untouched; each column of Y equals the oracle's SpMV of that column of X.
+    The reference's own leading dimension (ldx = ldy = n_rows, columns of X
+    overlapping when n_cols > n_rows) matches the oracle's column-major SpMM."""
+    B = matgen.lomax_band(9000, half=1000)
+    A = rectangular(B, n_rows=6000) if shape == "wide" else rectangular(B, n_cols=6000)
+    n, m = A.n_rows, A.n_cols
+    rp, col = ctx.upload(A.row_ptr), ctx.upload(A.col)
+    for k in (5, 32):
+        val, _ = operands(vt, A)
+        v = ctx.upload(val)
+        X = operands(vt, A, k, seed=40 + k)[1].reshape(k, m)
+        ldx, ldy = m + 7, n + 3
+        Xp = np.zeros((k, ldx), X.dtype)
+        Xp[:, :m] = X
+        dX = ctx.upload(Xp.ravel())
+        dY = ctx.upload(np.full(ldy * k, 7.0, X.dtype))
+        ctx.crs_spmm(vt, n, m, A.nnz, rp, col, v, k, False, dX, ldx, dY, ldy)
+        got = ctx.download(dY).reshape(k, ldy)
+        for c in range(k):
+            assert_bitwise(got[c, :n], oracle.crs_spmv(A.row_ptr, A.col, val, X[c]),
+                           f"{shape} k={k} column {c}")
+        assert (got[:, n:] == 7.0).all()
+        # the reference's layout: leading dimension n_rows for X and Y
+        need = (k - 1) * n + m
+        Xr = (matgen.uniform_f64(90 + k, need) if vt == capi.F64
+              else matgen.uniform_f32(90 + k, need))
+        dXr = ctx.upload(Xr)
+        dYr = ctx.zeros(n * k, X.dtype)
+        ctx.crs_spmm(vt, n, m, A.nnz, rp, col, v, k, False, dXr, n, dYr, n)
+        assert_bitwise(ctx.download(dYr), oracle.crs_spmm(A.row_ptr, A.col, val, k, False, Xr),
+                       f"{shape} k={k} ld = n_rows")
+        for d in (v, dX, dY, dXr, dYr):
+            d.free()
+    for d in (rp, col):
+        d.free()
+
+
+def test_crs_spmm_column_major_refuses_overlapping_y(ctx):
+    """A column-major Y whose columns would overlap (ldy < n_rows) is refused
+    before anything is launched; with k = 1 the leading dimension is unused."""
+    A = matgen.lomax_band(3000, half=300)
+    rp, col, v = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
+    k = 4
+    dX = ctx.upload(matgen.uniform_f64(3, A.n_cols * k))
+    dY = ctx.upload(np.full(A.n_rows * k, 5.0))
+    lib = ctx.lib
+    for ldy in (A.n_rows - 1, 0, -A.n_rows):
+        st = lib.smax_hip_crs_spmm(ctx.handle, capi.F64, A.n_rows, A.n_cols, A.nnz, rp.ptr,
+                                   col.ptr, v.ptr, k, 0, dX.ptr, A.n_rows, dY.ptr, ldy, None)
+        assert st == capi.ERROR_INVALID_VALUE, ldy
+        assert b"ldy" in lib.smax_hip_last_error(ctx.handle)
+    ctx.sync()
+    assert (ctx.download(dY) == 5.0).all()
+    st = lib.smax_hip_crs_spmm(ctx.handle, capi.F64, A.n_rows, A.n_cols, A.nnz, rp.ptr,
+                               col.ptr, v.ptr, 1, 0, dX.ptr, A.n_rows, dY.ptr, 0, None)
+    assert st == capi.SUCCESS
+    assert_bitwise(ctx.download(dY)[:A.n_rows],
+                   oracle.crs_spmv(A.row_ptr, A.col, A.val, ctx.download(dX)[:A.n_cols]))
+    for d in (rp, col, v, dX, dY):
+        d.free()
+
+
+def test_crs_spmm_column_major_graph_survives_workspace_growth(ctx):
+    """A HIP graph captured with a column-major SpMM keeps its staging
+    workspace: a later, larger product grows the context's workspace, and
+    every replay of the old graph (eager work and synchronisation between
+    replays) still equals the eager result.  A context of its own, so the
+    workspace starts empty whatever ran before."""
+    torch = pytest.importorskip("torch")
+    with capi.Context(0) as own:
+        _graph_survives_growth(own, torch)
+
+
+def _graph_survives_growth(ctx, torch):
+    A = matgen.lomax_band(5000, half=600)
+    k = 8
+    rp, col, v = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
+    X = matgen.uniform_f64(11, A.n_cols * k)
+    dX, dY = ctx.upload(X), ctx.zeros(A.n_rows * k, np.float64)
+    want = oracle.crs_spmm(A.row_ptr, A.col, A.val, k, False, X)
+    call = lambda: ctx.crs_spmm(capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, v, k, False,
+                                dX, A.n_rows, dY, A.n_rows)
+    st = torch.cuda.Stream()
+    ctx.set_stream(st.cuda_stream)
+    try:
+        call()  # sizes the workspace before capture
+        ctx.sync()
+        assert_bitwise(ctx.download(dY), want, "eager")
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=st):
+            call()
+        # a larger product (k = 64 on a bigger matrix) grows the workspace
+        B = matgen.lomax_band(40000, half=5000)
+        kb = 64
+        brp, bcol, bv = ctx.upload(B.row_ptr), ctx.upload(B.col), ctx.upload(B.val)
+        bX = matgen.uniform_f64(12, B.n_cols * kb)
+        dbX, dbY = ctx.upload(bX), ctx.zeros(B.n_rows * kb, np.float64)
+        ctx.crs_spmm(capi.F64, B.n_rows, B.n_cols, B.nnz, brp, bcol, bv, kb, False, dbX,
+                     B.n_rows, dbY, B.n_rows)
+        ctx.sync()
+        assert_bitwise(ctx.download(dbY), oracle.crs_spmm(B.row_ptr, B.col, B.val, kb, False, bX),
+                       "larger product")
+        for rep in range(3):
+            ctx._check(ctx.lib.smax_hip_zero(ctx.handle, dY.ptr, dY.nbytes))
+            other = torch.ones(1 << 20, device="cuda") * (rep + 1)  # eager work between replays
+            with torch.cuda.stream(st):
+                g.replay()
+            torch.cuda.synchronize()
+            assert float(other.sum()) == float((rep + 1) << 20)
+            assert_bitwise(ctx.download(dY), want, f"replay {rep}")
+        for d in (brp, bcol, bv, dbX, dbY):
+            d.free()
+    finally:
+        ctx.set_stream(None)
+    for d in (rp, col, v, dX, dY):
+        d.free()
