@@ -11,12 +11,22 @@ the average kernel time measured with HIP events on the library's stream.
 
 Also reported: every (C, sigma) variant, CRS (lane-per-row and LDS row-block
 schedules), rocSPARSE's CSR SpMV algorithms on the same arrays (library
-baseline, comparison only), FP32/BF16-value SELL, the 512^2 5-point Laplacian (warm and cold L2/MALL), FP64 CG vs
-BF16/FP32-value refined CG on the 160^3 27-point stencil, FP32 SpMM k=32 with
-its adjoint and SDDMM (1M-row Lomax band), BCRS 4x4 BF16 SpMM forward and
+baseline, comparison only), FP32/BF16-value SELL, the 512^2 5-point
+Laplacian (warm and cold L2/MALL, and the kernel-only cold duration from a
+committed rocprofv3 trace), FP64 CG vs BF16/FP32-value refined CG on the
+160^3 27-point stencil (with first-solve setup), FP32 SpMM k=32 with its
+adjoint and SDDMM (1M-row Lomax band), BCRS 4x4 BF16 SpMM forward and
 adjoint (96^3 7-point block adjacency, k = 8 and 64), the device copy
-bandwidth used as the measured stream peak, and the oracle's OpenMP CRS SpMV
-on the host cores (bounded sample) as the CPU baseline.
+bandwidth by several methods (the best is the measured stream peak), and the
+oracle's OpenMP kernels on the host cores: the CRS SpMV of the headline
+matrix (bounded sample) as the CPU baseline, and one kernel per config.
+Every FP32/BF16 result carries its error against the FP64 product and the
+bound that error must stay under.
+
+    python bench.py --summarize-trace DIR
+
+prints per-kernel durations from rocprofv3 --kernel-trace CSV output instead
+(no GPU work); profiles/c1_poisson512_cold_warm_kernels.json is made this way.
 """
 from __future__ import annotations
 
@@ -31,6 +41,11 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+
+# the CPU baseline's OpenMP threads: bound close, one per core (read when the
+# oracle's OpenMP runtime starts, which is after this point)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 # torch first: it brings its own HIP runtime, and libsmax_hip.so then binds to
 # the same one (the C1 warm figure replays a torch-captured HIP graph).
@@ -49,6 +64,26 @@ def log(msg: str) -> None:
 
 def b_alg(n_rows, n_cols, nnz, vb=8, ib=4, xb=8, yb=8) -> int:
     return nnz * (vb + ib) + (n_rows + 1) * ib + n_cols * xb + n_rows * yb
+
+
+U32 = 2.0 ** -24
+U_VALUE = {"f32": U32, "bf16": 2.0 ** -8 + 2.0 ** -24 + 2.0 ** -32}
+
+
+def err_vs_fp64(y_lo, y64, scale, n_max, u_v) -> dict:
+    """Largest |y_lo - y_64| / (|A| |x|) over the outputs, next to its bound
+    u_v + gamma_n (1 + u_v) (u_v: relative rounding of a stored value;
+    gamma_n = n u / (1 - n u), u = 2^-24, n the longest chain).  y_64 is
+    the FP64 product of the FP64 matrix with the same (exactly widened)
+    operands, computed by the FP64 kernels, which the tests show
+    bit-identical to the oracle."""
+    y_lo = np.asarray(y_lo, np.float64).ravel()
+    y64, scale = np.asarray(y64).ravel(), np.asarray(scale).ravel()
+    nz = scale > 0
+    ratio = float(np.max(np.abs(y_lo[nz] - y64[nz]) / scale[nz])) if nz.any() else 0.0
+    g = n_max * U32 / (1 - n_max * U32)
+    return {"max_rel_err_vs_fp64": ratio, "bound": u_v + g * (1 + u_v),
+            "metric": "max |y_lo - y_64| / (|A||x|) per output"}
 
 
 def time_launches(ctx, fn, steps, warmup) -> float:
@@ -97,20 +132,33 @@ def graph_ms(ctx, fn, per_graph=200, replays=10, check=None) -> float:
 
 
 def stream_peak(ctx, steps=10) -> dict:
+    """Device copy bandwidth by several methods over 4 GiB arrays: the copy
+    kernel with 1, 2, 4 and 8 16-byte loads in flight per lane, and
+    hipMemcpy device-to-device; each the best of `steps` event-timed runs
+    (read + write bytes).  The best of them is the measured stream peak that
+    every frac_of_measured_stream divides by.  The triad is reported too."""
     n = 1 << 29  # 4 GiB per array
     a = ctx.zeros(n, np.float64)
     b = ctx.zeros(n, np.float64)
-    ms = min(time_launches(ctx, lambda: ctx.stream_copy(a, b, n), 1, 2)
-             for _ in range(steps))
-    copy_gbs = 2 * n * 8 / ms / 1e6
+    gbs = lambda ms, arrays=2: round(arrays * n * 8 / ms / 1e6, 1)
+    methods = {}
+    ms = min(time_launches(ctx, lambda: ctx.stream_copy(a, b, n), 1, 2) for _ in range(steps))
+    methods["copy_kernel"] = gbs(ms)
+    for L in (2, 4, 8):
+        ms = min(time_launches(ctx, lambda: ctx.stream_copy_wide(a, b, n, L), 1, 2)
+                 for _ in range(steps))
+        methods[f"copy_wide_kernel_{L}_loads"] = gbs(ms)
+    d2d = lambda: ctx._check(ctx.lib.smax_hip_memcpy_d2d(ctx.handle, b.ptr, a.ptr, n * 8))
+    ms = min(time_launches(ctx, d2d, 1, 2) for _ in range(steps))
+    methods["hipMemcpy_d2d"] = gbs(ms)
     c = ctx.zeros(n, np.float64)
     ms3 = min(time_launches(ctx, lambda: ctx.stream_triad(a, b, c, 0.5, n), 1, 2)
               for _ in range(steps))
-    triad_gbs = 3 * n * 8 / ms3 / 1e6
     for d in (a, b, c):
         d.free()
-    return {"copy_GBs": round(copy_gbs, 1), "triad_GBs": round(triad_gbs, 1),
-            "array_GiB": 4}
+    best = max(methods, key=methods.get)
+    return {"copy_GBs": methods[best], "copy_method": best, "copy_methods_GBs": methods,
+            "triad_GBs": gbs(ms3, 3), "array_GiB": 4}
 
 
 def gather_ceiling(ctx, A, steps, warmup) -> dict:
@@ -151,80 +199,213 @@ def hbm_traffic(kernel: str, n_rows: int, fmt: str):
     return None if rec is None else rec["bytes_per_launch"]
 
 
-def cpu_baseline(A, x, seconds) -> dict:
-    """The oracle's OpenMP CRS SpMV (reference loop order) on host cores."""
-    from oracle import oracle
-    y = np.empty(A.n_rows)
-    oracle.crs_spmv(A.row_ptr, A.col, A.val, x)  # warm-up
-    n_iter, runtime = 1, 0.0
+def doubling_time(fn, min_seconds) -> tuple[float, int, float]:
+    """Seconds per call of fn: one warm-up call, then a doubling iteration
+    count until one timed batch lasts >= min_seconds (the method of the
+    reference's benchmarks, examples/benchmarks/benchmarks_common.hpp:60-95).
+    Returns (seconds per call, calls in the last batch, its duration)."""
+    fn()
+    n = 1
     while True:
         t0 = time.perf_counter()
-        for _ in range(n_iter):
-            y = oracle.crs_spmv(A.row_ptr, A.col, A.val, x)
-        runtime = time.perf_counter() - t0
-        if runtime >= seconds or n_iter >= 1 << 20:
-            break
-        n_iter *= 2
-    t = runtime / n_iter
+        for _ in range(n):
+            fn()
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds or n >= 1 << 20:
+            return dt / n, n, dt
+        n *= 2
+
+
+def host_description() -> dict:
+    """The host the CPU baseline ran on: lscpu's model, sockets, cores and
+    NUMA nodes, the CPUs this process may use, and the OpenMP settings."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket",
+                               "Thread(s) per core", "NUMA node(s)", "CPU(s)"):
+                info[key.strip()] = val.strip()
+    except (OSError, subprocess.SubprocessError) as e:
+        info["lscpu_error"] = str(e)
+    info["nproc"] = os.cpu_count()
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    for var in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES"):
+        info[var] = os.environ.get(var)
+    return info
+
+
+def cpu_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(A, x, seconds) -> dict:
+    """The oracle's OpenMP CRS SpMV (the reference's loop order and fma
+    chain, spmv_cpu_crs_impl.hpp:9-38) on the host cores, on the headline
+    matrix: the bounded sample the headline's cpu_baseline reports."""
+    from oracle import oracle
+    y = np.empty(A.n_rows)
+    t, n_iter, runtime = doubling_time(
+        lambda: oracle.crs_spmv(A.row_ptr, A.col, A.val, x, out=y), seconds)
     gbs = b_alg(A.n_rows, A.n_cols, A.nnz) / t / 1e9
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(gbs, 2), "unit": "GB/s", "cores": cores,
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": cpu_threads(),
             "kind": "port",
             "sample": f"{n_iter} CRS SpMVs of the N=4M Lomax matrix "
                       f"({runtime:.1f} s, {t * 1e3:.2f} ms each, "
-                      f"{2 * A.nnz / t / 1e9:.2f} GFLOP/s), OpenMP static rows"}
+                      f"{2 * A.nnz / t / 1e9:.2f} GFLOP/s), OpenMP static rows, "
+                      f"doubling loop after a warm-up call"}
+
+
+def cpu_configs(ctx, A, x, seconds) -> dict:
+    """The oracle's OpenMP kernels (reference loop orders) on every config,
+    each timed by the doubling loop for >= `seconds`; GFLOP/s and GB/s on
+    the same algorithmic bytes as the GPU lines.
+      C1  CRS SpMV, 512^2 5-point Laplacian, FP64
+      C2  SELL-64-4096 SpMV of the 4M-row Lomax matrix, FP64 (arrays from
+          the GPU conversion, bit-identical to the reference's)
+      C3  CRS SpMV of the 160^3 27-point stencil, FP64: the product of one
+          CG iteration
+      C4  CRS SpMM, FP32, k = 32 row-major, 1M-row Lomax matrix
+      C5  BCRS 4x4 SpMM, BF16 values widened to FP32, k = 8, 96^3 block
+          adjacency (the reference CPU path runs k BCRS SpMVs; the oracle's
+          k-column loop is that, with the columns innermost)"""
+    from oracle import oracle
+    out = {"host": host_description(), "threads": cpu_threads(), "kind": "port",
+           "method": "warm-up call, then doubling iteration counts until one batch "
+                     f"lasts >= {seconds} s (benchmarks_common.hpp:60-95)"}
+
+    def record(name, fn, balg, flops, what):
+        t, n, dt = doubling_time(fn, seconds)
+        out[name] = {"ms": round(t * 1e3, 4), "GBs_alg": round(balg / t / 1e9, 2),
+                     "GFLOPs": round(flops / t / 1e9, 2), "calls": n,
+                     "batch_s": round(dt, 2), "what": what}
+        log(f"CPU {name}: {out[name]}")
+
+    P = matgen.poisson2d(512)
+    px = matgen.uniform_f64(matgen.SEED_X, P.n_cols)
+    py = np.empty(P.n_rows)
+    record("c1_poisson512_crs_f64", lambda: oracle.crs_spmv(P.row_ptr, P.col, P.val, px, out=py),
+           b_alg(P.n_rows, P.n_cols, P.nnz), 2 * P.nnz, "CRS SpMV")
+    del P, px, py
+
+    rp, col, val = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
+    s = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 8, 64, 4096)
+    arr = s.arrays(np.float64)
+    nch = s.info.n_chunks
+    s.destroy()
+    for d in (rp, col, val):
+        d.free()
+    sy = np.empty(nch * 64)
+    record("c2_lomax4M_sell64_4096_f64",
+           lambda: oracle.scs_spmv(64, nch, arr["chunk_ptr"], arr["chunk_lengths"],
+                                   arr["col"], arr["val"], x, out=sy),
+           b_alg(A.n_rows, A.n_cols, A.nnz), 2 * A.nnz, "SELL-C-sigma SpMV")
+    del arr, sy
+
+    S = matgen.stencil27(160)
+    sx = matgen.uniform_f64(matgen.SEED_B, S.n_cols)
+    sy = np.empty(S.n_rows)
+    record("c3_stencil27_160_crs_f64", lambda: oracle.crs_spmv(S.row_ptr, S.col, S.val, sx, out=sy),
+           b_alg(S.n_rows, S.n_cols, S.nnz), 2 * S.nnz, "CRS SpMV (one CG iteration's product)")
+    del S, sx, sy
+
+    k = 32
+    M = matgen.lomax_band(1_000_000)
+    v32 = M.val.astype(np.float32)
+    X = matgen.uniform_f32(matgen.SEED_X, M.n_cols * k)
+    Y = np.empty(M.n_rows * k, np.float32)
+    record("c4_lomax1M_crs_spmm_f32_k32",
+           lambda: oracle.crs_spmm(M.row_ptr, M.col, v32, k, True, X, out=Y),
+           M.nnz * 8 + (M.n_rows + 1) * 4 + (M.n_rows + M.n_cols) * k * 4, 2 * M.nnz * k,
+           "CRS SpMM, row-major X/Y")
+    del M, v32, X, Y
+
+    k = 8
+    B = matgen.block7(96, 4)
+    brp, bcol, bval = ctx.upload(B.row_ptr), ctx.upload(B.col), ctx.upload(matgen.to_bf16(B.val))
+    H = ctx.bcrs_from_crs(B.n_rows, B.n_cols, B.nnz, brp, bcol, bval, 2, 4, 4, 4, 4, False)
+    for d in (brp, bcol, bval):
+        d.free()
+    h = H.arrays(np.uint16)
+    nbr, nb = H.info.n_block_rows, H.info.n_blocks
+    H.destroy()
+    X = matgen.uniform_f32(matgen.SEED_X, B.n_rows * k)
+    Y = np.empty(B.n_rows * k, np.float32)
+    record("c5_block7_96_bcrs_spmm_bf16_k8",
+           lambda: oracle.bcrs_spmm(nbr, 4, 4, 4, 4, h["col"], h["row_ptr"], h["val"], False, k,
+                                    X, out=Y),
+           nb * (16 * 2 + 4) + (nbr + 1) * 4 + 2 * B.n_rows * k * 4, 2 * nb * 16 * k,
+           "BCRS 4x4 SpMM, BF16 values widened to FP32")
+    return out
 
 
 def cg_section(ctx) -> dict:
     """27-point stencil on 160^3 (N = 4,096,000, nnz = 109,215,352, diagonal
-    26.01): FP64 CG vs iterative refinement with BF16 values (FP32 diagonal
-    side array) to ||b - A x|| / ||b|| <= 1e-10, b uniform [-1, 1).  Every
-    solution's residual is recomputed on the host in FP64 (scipy's CSR
-    product, independent of the GPU kernels)."""
+    26.01): FP64 CG vs iterative refinement with BF16 and FP32 values (FP32
+    diagonal side array) to ||b - A x|| / ||b|| <= 1e-10, b uniform [-1, 1).
+    Each solver runs on a SELL handle of its own, so its first solve pays the
+    one-time setup (solver copies of the values, column tables): reported as
+    setup_seconds of the first solve, next to the best of three solves.
+    Every solution's residual is recomputed on the host in FP64 (scipy's CSR
+    product, independent of the GPU kernels), and the refined solutions'
+    distance from the FP64 CG solution is stated."""
     import scipy.sparse as sp
     log("C3: 27-point stencil 160^3")
     A = matgen.stencil27(160)
     b = matgen.uniform_f64(matgen.SEED_B, A.n_rows)
     rp, col, val = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
-    S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 8, 64, 1)
     db = ctx.upload(b)
-    out = {"n_rows": A.n_rows, "nnz": A.nnz, "format": "SELL-64-1",
-           "fill": round(S.info.n_elements / A.nnz, 4)}
+    out = {"n_rows": A.n_rows, "nnz": A.nnz, "format": "SELL-64-1"}
     Ah = sp.csr_matrix((A.val, A.col, A.row_ptr), shape=(A.n_rows, A.n_cols))
 
     def solve(mixed, lo=capi.BF16, inner=0.1, reps=3):
-        best = None
+        S = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, val, 8, 64, 1)
+        out["fill"] = round(S.info.n_elements / A.nnz, 4)
+        best = first = None
         for _ in range(reps):
             dx = ctx.zeros(A.n_rows, np.float64)
             prm = capi.CgParams(rel_tol=1e-10, max_iters=20000, check_every=25,
                                 inner_rel_tol=inner, max_inner=5000,
                                 inner_value_type=lo, keep_diag_f32=1)
             rep = ctx.cg(S, db, dx, prm, mixed)
+            first = first or (rep.seconds, rep.setup_seconds)
             if best is None or rep.seconds < best[0].seconds:
                 best = (rep, ctx.download(dx))
             dx.free()
+        S.destroy()
         rep, x = best
         r = b - Ah @ x
         return {"status": rep.status, "seconds": round(rep.seconds, 5),
                 "iterations": rep.iterations, "outer": rep.outer_iterations,
                 "rel_residual_gpu": rep.rel_residual,
                 "rel_residual_host_fp64": float(np.linalg.norm(r) / np.linalg.norm(b)),
-                "setup_seconds": round(rep.setup_seconds, 5),
+                "first_solve_seconds": round(first[0], 5),
+                "first_solve_setup_seconds": round(first[1], 5),
+                "setup_seconds_cached": round(rep.setup_seconds, 5),
                 "column_coding": {capi.COLUMNS_8: "8-bit table indices",
                                   capi.COLUMNS_16: "16-bit offsets",
                                   capi.COLUMNS_32: "32-bit columns"}.get(rep.column_coding)}, x
 
     fp64, x64 = solve(False)
     bf16, xbf = solve(True, capi.BF16)
-    f32, _ = solve(True, capi.F32)
-    bf16["max_abs_diff_vs_fp64_cg"] = float(np.max(np.abs(xbf - x64)))
+    f32, x32 = solve(True, capi.F32)
+    n64 = np.linalg.norm(x64)
+    for rec, x in ((bf16, xbf), (f32, x32)):
+        rec["max_abs_diff_vs_fp64_cg"] = float(np.max(np.abs(x - x64)))
+        rec["rel_l2_diff_vs_fp64_cg"] = float(np.linalg.norm(x - x64) / n64)
+    with_setup = lambda r: r["first_solve_seconds"] + r["first_solve_setup_seconds"]
     out.update({"fp64_cg": fp64, "ir_bf16": bf16, "ir_f32": f32,
                 "speedup_bf16_vs_fp64": round(fp64["seconds"] / bf16["seconds"], 3),
-                "speedup_f32_vs_fp64": round(fp64["seconds"] / f32["seconds"], 3)})
+                "speedup_f32_vs_fp64": round(fp64["seconds"] / f32["seconds"], 3),
+                "speedup_bf16_vs_fp64_first_solve_incl_setup":
+                    round(with_setup(fp64) / with_setup(bf16), 3),
+                "speedup_f32_vs_fp64_first_solve_incl_setup":
+                    round(with_setup(fp64) / with_setup(f32), 3)})
     log(f"C3 {out}")
     for d in (rp, col, val, db):
         d.free()
-    S.destroy()
     return out
 
 
@@ -331,25 +512,42 @@ def vendor_section(ctx, A, rp, col, val, dx, steps, warmup) -> dict:
     return out
 
 
-def lowp_section(ctx, A, rp, col, steps, warmup) -> dict:
+def lowp_section(ctx, A, rp, col, val, steps, warmup) -> dict:
     """The 4M-row Lomax matrix with FP32 and BF16 values (FP32 x, y and
-    accumulation), SELL-64 at sigma = 4096 and 65536 through the handle path."""
+    accumulation), SELL-64 at sigma = 4096 and 65536 through the handle path,
+    each with its error against the FP64 product."""
     out = {}
+    lens = np.diff(A.row_ptr)
+    dabs = ctx.upload(np.abs(A.val))
     for name, vt, vals, vb in (("f32", capi.F32, A.val.astype(np.float32), 4),
                                ("bf16", capi.BF16, matgen.to_bf16(A.val), 2)):
         ba = A.nnz * (vb + 4) + (A.n_rows + 1) * 4 + A.n_cols * 4 + A.n_rows * 4
         dv = ctx.upload(vals)
-        dx = ctx.upload(matgen.uniform_f32(matgen.SEED_X, A.n_cols))
+        x32 = matgen.uniform_f32(matgen.SEED_X, A.n_cols)
+        dx = ctx.upload(x32)
+        # FP64 reference: the FP64 matrix times x widened exactly, and |A||x|
+        x64, xa = ctx.upload(x32.astype(np.float64)), ctx.upload(np.abs(x32.astype(np.float64)))
+        y64, ya = ctx.zeros(A.n_rows, np.float64), ctx.zeros(A.n_rows, np.float64)
+        ctx.crs_spmv(capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, val, x64, y64, None)
+        ctx.crs_spmv(capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, dabs, xa, ya, None)
+        ref, scale = ctx.download(y64), ctx.download(ya)
+        for d in (x64, xa, y64, ya):
+            d.free()
         for sigma in (4096, 65536):
             s = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, dv, vb, 64, sigma)
             yy = ctx.zeros(s.info.n_rows_padded, np.float32)
             ms = time_launches(ctx, lambda: s.apply(vt, dx, yy), steps, warmup)
-            out[f"{name}_C64_s{sigma}"] = {"ms": round(ms, 4), "B_alg": ba,
-                                           "GBs_alg": round(ba / ms / 1e6, 1)}
+            yo = ctx.zeros(A.n_rows, np.float32)
+            s.apply(vt, dx, yo, unpermuted=True)
+            out[f"{name}_C64_s{sigma}"] = {
+                "ms": round(ms, 4), "B_alg": ba, "GBs_alg": round(ba / ms / 1e6, 1),
+                **err_vs_fp64(ctx.download(yo), ref, scale, int(lens.max()), U_VALUE[name])}
+            yo.free()
             yy.free()
             s.destroy()
         dv.free()
         dx.free()
+    dabs.free()
     log(f"SELL low precision {out}")
     return out
 
@@ -394,8 +592,10 @@ def c4_section(ctx, steps, warmup) -> dict:
         ms = time_launches(ctx, fn, steps, warmup)
         out[name] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1),
                      "GFLOPs": round(2 * A.nnz * k / ms / 1e6, 1)}
-    out["fd_check"] = fd_check(A, v32, X, W, k, ctx.download(ddv).astype(np.float64),
-                               ctx.download(dG).astype(np.float64).reshape(A.n_cols, k))
+    from tests.helpers import fd_gradient_check
+    out["fd_check"] = fd_gradient_check(A, v32, X, W, k, ctx.download(ddv).astype(np.float64),
+                                        ctx.download(dG).astype(np.float64).reshape(A.n_cols, k))
+    out["error_vs_fp64"] = c4_errors(ctx, A, rp, col, X, W, k, runs, dY, dG, ddv)
     lib = vendor_lib()
     if not isinstance(lib, str):
         runs["forward_crs"]()
@@ -414,52 +614,58 @@ def c4_section(ctx, steps, warmup) -> dict:
     return out
 
 
-def fd_check(A, v32, Xf, Wf, k, gv, gx, n_samples=64, seed=2024) -> dict:
-    """FP64 central differences of L = sum(Y o W), Y = A X, for n_samples
-    value entries and n_samples X entries, against the SDDMM (gv, CRS order)
-    and A^T W (gx) gradients.  L is linear in each entry, so the central
-    difference is exact up to rounding; it is evaluated on the rows the entry
-    touches (every other term of L cancels).  Metric per entry:
-    |g - g_FD| / max(|g_FD|, sum of |terms|)."""
-    rng = np.random.default_rng(seed)
-    X = Xf.astype(np.float64).reshape(A.n_cols, k)
-    W = Wf.astype(np.float64).reshape(A.n_rows, k)
-    v64 = v32.astype(np.float64)
-    rows_of = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
-
-    def loss(vals, Xm, rows):
-        tot = 0.0
-        for i in rows:
-            s, e = A.row_ptr[i], A.row_ptr[i + 1]
-            tot += float((vals[s:e] @ Xm[A.col[s:e]]) @ W[i])
-        return tot
-
-    def central(arr, at, rows, vals, Xm):
-        # perturb arr[at] by +-h in place (arr is vals or Xm), then restore
-        keep = arr[at]
-        h = 2.0 ** -10 * max(1.0, abs(keep))
-        arr[at] = keep + h
-        lp = loss(vals, Xm, rows)
-        arr[at] = keep - h
-        lm = loss(vals, Xm, rows)
-        arr[at] = keep
-        return (lp - lm) / (2 * h)
-
-    worst = 0.0
-    for j in rng.choice(A.nnz, n_samples, replace=False):
-        i = rows_of[j]
-        fd = central(v64, j, [i], v64, X)
-        scale = max(abs(fd), float(np.abs(W[i] * X[A.col[j]]).sum()))
-        worst = max(worst, abs(gv[j] - fd) / scale)
-    for idx in rng.choice(A.n_cols * k, n_samples, replace=False):
-        jr, vv = divmod(int(idx), k)
-        mask = A.col == jr
-        rows = np.unique(rows_of[mask])
-        fd = central(X, (jr, vv), rows, v64, X)
-        scale = max(abs(fd), float(np.abs(v64[mask] * W[rows_of[mask], vv]).sum()))
-        worst = max(worst, abs(gx[jr, vv] - fd) / scale)
-    return {"entries": 2 * n_samples, "max_rel_err": float(worst), "bound": 1e-6,
-            "metric": "|g - g_FD| / max(|g_FD|, sum |terms|), FP64 central differences"}
+def c4_errors(ctx, A, rp, col, X, W, k, runs, dY, dG, ddv) -> dict:
+    """FP32 results of C4 against FP64: Y = A X and A^T W against the FP64
+    matrix's products (FP64 kernels, operands widened exactly), the SDDMM
+    against FP64 sums on 1M sampled entries (the SDDMM rounds no stored
+    value; its bound is its reduction tree's, 7u for k = 32)."""
+    out = {}
+    lens = np.diff(A.row_ptr)
+    X64, W64 = X.astype(np.float64), W.astype(np.float64)
+    v64, va = ctx.upload(A.val), ctx.upload(np.abs(A.val))
+    dX64, dXa = ctx.upload(X64), ctx.upload(np.abs(X64))
+    Y64, Ya = ctx.zeros(A.n_rows * k, np.float64), ctx.zeros(A.n_rows * k, np.float64)
+    # the planned and column-major CRS products are bit-identical to these
+    # two (tests/test_gpu_fullsize.py)
+    for name in ("forward_crs", "forward_sell"):
+        runs[name]()
+        y_lo = ctx.download(dY)
+        if name == "forward_crs":
+            ctx.crs_spmm(capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, v64, k, True, dX64, k,
+                         Y64, k)
+            ctx.crs_spmm(capi.F64, A.n_rows, A.n_cols, A.nnz, rp, col, va, k, True, dXa, k,
+                         Ya, k)
+            ref, scale = ctx.download(Y64), ctx.download(Ya)
+        out[name] = err_vs_fp64(y_lo, ref, scale, int(lens.max()), U32)
+    for d in (dX64, dXa, Y64, Ya):
+        d.free()
+    T64 = ctx.scs_transpose_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v64, 8, 64, A.n_cols)
+    dW64, dWa = ctx.upload(W64), ctx.upload(np.abs(W64))
+    G64 = ctx.zeros(A.n_cols * k, np.float64)
+    runs["adjoint_sell_AT"]()
+    g_lo = ctx.download(dG)
+    T64.apply_spmm(capi.F64, k, dW64, G64)
+    ref = ctx.download(G64)
+    T64.refresh_values(va)
+    T64.apply_spmm(capi.F64, k, dWa, G64)
+    scale = ctx.download(G64)
+    col_len = np.bincount(A.col, minlength=A.n_cols)
+    out["adjoint_sell_AT"] = err_vs_fp64(g_lo, ref, scale, int(col_len.max()), U32)
+    for d in (v64, va, dW64, dWa, G64):
+        d.free()
+    T64.destroy()
+    runs["sddmm_sell"]()
+    dv = ctx.download(ddv).astype(np.float64)
+    ent = np.random.default_rng(7).choice(A.nnz, 1_000_000, replace=False)
+    rows = np.repeat(np.arange(A.n_rows), lens)[ent]
+    Wr, Xr = W64.reshape(-1, k)[rows], X64.reshape(-1, k)[A.col[ent]]
+    exact = np.einsum("ij,ij->i", Wr, Xr)
+    scale = np.einsum("ij,ij->i", np.abs(Wr), np.abs(Xr))
+    e = err_vs_fp64(dv[ent], exact, scale, 0, 0.0)
+    e["bound"] = 7 * U32 * (1 + 4 * U32) + k * 2.0 ** -53
+    e["sample"] = "1M random stored entries"
+    out["sddmm_sell"] = e
+    return out
 
 
 def c5_section(ctx, steps, warmup) -> dict:
@@ -484,19 +690,36 @@ def c5_section(ctx, steps, warmup) -> dict:
                    "GFLOPs": round(2 * nb * 16 / ms / 1e6, 1), "B_alg": ba1}
     x1.free()
     y1.free()
+    # the FP64 matrix (and |A|) in the same BCRS layout, for the errors vs FP64
+    rp, col = ctx.upload(A.row_ptr), ctx.upload(A.col)
+    v64, va = ctx.upload(A.val), ctx.upload(np.abs(A.val))
+    B64 = ctx.bcrs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v64, 8, 4, 4, 4, 4, False)
+    Ba = ctx.bcrs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, va, 8, 4, 4, 4, 4, False)
+    for d in (rp, col, v64, va):
+        d.free()
+    T64, Ta = B64.transpose(), Ba.transpose()
+    n_chain = 4 * int(np.diff(B.arrays(np.uint16)["row_ptr"]).max())
     for k in (8, 64):
         ba = nb * (16 * 2 + 4) + (B.info.n_block_rows + 1) * 4 + 2 * A.n_rows * k * 4
-        X = ctx.upload(matgen.uniform_f32(matgen.SEED_X, A.n_rows * k))
+        Xh = matgen.uniform_f32(matgen.SEED_X, A.n_rows * k)
+        X = ctx.upload(Xh)
         Y = ctx.zeros(A.n_rows * k, np.float32)
-        for name, M in (("forward", B), ("adjoint_AT", T)):
+        X64, Xa = ctx.upload(Xh.astype(np.float64)), ctx.upload(np.abs(Xh.astype(np.float64)))
+        Y64 = ctx.zeros(A.n_rows * k, np.float64)
+        for name, M, M64, Mabs in (("forward", B, B64, Ba), ("adjoint_AT", T, T64, Ta)):
             ms = time_launches(ctx, lambda: M.spmm(capi.BF16, k, X, Y), steps, warmup)
+            M64.spmm(capi.F64, k, X64, Y64)
+            ref = ctx.download(Y64)
+            Mabs.spmm(capi.F64, k, Xa, Y64)
             out[f"{name}_k{k}"] = {"ms": round(ms, 4), "GBs_alg": round(ba / ms / 1e6, 1),
                                    "GFLOPs": round(2 * nb * 16 * k / ms / 1e6, 1),
-                                   "B_alg": ba}
-        X.free()
-        Y.free()
-    B.destroy()
-    T.destroy()
+                                   "B_alg": ba,
+                                   **err_vs_fp64(ctx.download(Y), ref, ctx.download(Y64),
+                                                 n_chain, U_VALUE["bf16"])}
+        for d in (X, Y, X64, Xa, Y64):
+            d.free()
+    for M in (B, T, B64, Ba, T64, Ta):
+        M.destroy()
     # rocSPARSE has no BF16-value x FP32-vector SpMM: compare both in FP32
     lib = vendor_lib()
     if not isinstance(lib, str):
@@ -524,6 +747,55 @@ def c5_section(ctx, steps, warmup) -> dict:
     return out
 
 
+def summarize_trace(directory) -> dict:
+    """Per-kernel durations (us) from rocprofv3 --kernel-trace CSV files
+    under `directory`.  A launch that directly follows a copy_kernel on the
+    GPU timeline is 'cold' (the benchmark's C1 cold launches each follow a
+    1 GiB copy that evicts L2 and MALL); the others are 'warm'."""
+    import csv
+    rows = []
+    for f in sorted(Path(directory).rglob("*kernel_trace.csv")):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                             r["Kernel_Name"]))
+    rows.sort()
+    stats: dict = {}
+    prev = ""
+    for t0, t1, name in rows:
+        short = name.split("(")[0].replace("void ", "").strip()
+        kind = "cold" if prev.startswith("smaxhip::copy_kernel") else "warm"
+        stats.setdefault(short, {"cold": [], "warm": []})[kind].append((t1 - t0) / 1e3)
+        prev = short
+    out = {}
+    for name, d in stats.items():
+        out[name] = {k: {"count": len(v), "median_us": round(float(np.median(v)), 3),
+                         "min_us": round(float(np.min(v)), 3),
+                         "mean_us": round(float(np.mean(v)), 3)}
+                     for k, v in d.items() if v}
+    return out
+
+
+def c1_cold_kernel(kernel_prefix: str, b_alg_bytes: int, peak_gbs) -> dict | None:
+    """Kernel-only duration of C1's cold launches, from the committed
+    rocprofv3 kernel trace summary (profiles/c1_poisson512_cold_warm_kernels.json,
+    written by --summarize-trace), as us and as a fraction of the stream rate."""
+    f = ROOT / "profiles" / "c1_poisson512_cold_warm_kernels.json"
+    if not f.exists():
+        return None
+    summ = json.loads(f.read_text())["kernels"]
+    for name, rec in summ.items():
+        if name.split("<")[0].endswith(kernel_prefix) and "cold" in rec:
+            us = rec["cold"]["median_us"]
+            gbs = b_alg_bytes / us / 1e3
+            return {"cold_kernel_us": us, "cold_kernel_GBs": round(gbs, 1),
+                    "cold_kernel_frac_of_measured_stream":
+                        round(gbs / peak_gbs, 3) if peak_gbs else None,
+                    "warm_kernel_us": rec.get("warm", {}).get("median_us"),
+                    "source": f"rocprofv3 kernel trace, {f.name}"}
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -531,18 +803,26 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--n", type=int, default=4_000_000)
-    ap.add_argument("--full-sweep", action="store_true",
-                    help="also time sigma in {1, 64} (17x padded, 13 GB each)")
-    ap.add_argument("--sigmas", default="4096,65536,N",
-                    help="comma list of SELL sigma values (N = n_rows)")
+    ap.add_argument("--sigmas", default="1,64,4096,65536,N",
+                    help="comma list of SELL sigma values (N = n_rows); 1 and 64 are "
+                         "17x / 9-11x padded (13 GB each)")
     ap.add_argument("--cs", default="64,32", help="comma list of SELL chunk sizes")
     ap.add_argument("--mtx", default="",
                     help="also time FP64 SpMV on this Matrix Market file (CRS row "
                          "blocks and SELL-64 at sigma 1 and 4096), reported under extra")
+    ap.add_argument("--cpu-config-seconds", type=float, default=2.0,
+                    help="doubling-loop batch length of each per-config CPU baseline")
+    ap.add_argument("--summarize-trace", default="",
+                    help="print per-kernel durations from a rocprofv3 --kernel-trace "
+                         "CSV directory (cold launches: those right after a copy_kernel) "
+                         "and exit; runs no GPU work")
     ap.add_argument("--only", default="",
                     help="comma list of sections to run: "
                          "stream,sell,gather,vendor,lowp,crs,c1,cg,c4,c5,cpu")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(json.dumps(summarize_trace(args.summarize_trace), indent=1))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only; there is no multi-GPU path (rows "
               "would shard as contiguous row blocks with one x halo exchange per "
@@ -572,7 +852,6 @@ def main() -> int:
     sell = {}
     if want("sell"):
         sigmas = [A.n_rows if t == "N" else int(t) for t in args.sigmas.split(",")]
-        sigmas += [1, 64] if args.full_sweep else []
         for C_ in (int(t) for t in args.cs.split(",")):
             for sigma in sigmas:
                 t0 = time.perf_counter()
@@ -612,7 +891,7 @@ def main() -> int:
                                                     args.warmup)
 
     if want("lowp"):
-        extra["sell_lowp"] = lowp_section(ctx, A, rp, col, args.steps, args.warmup)
+        extra["sell_lowp"] = lowp_section(ctx, A, rp, col, val, args.steps, args.warmup)
 
     crs = {}
     if want("crs"):
@@ -664,12 +943,16 @@ def main() -> int:
                 "cold_method": "1 GiB copy between launches, median of 20 event-timed"}
 
         extra["c1_poisson512_f64_sell64_1"] = c1_times(lambda: ps.apply(capi.F64, pdx, pdy))
+        extra["c1_poisson512_f64_sell64_1"]["kernel_trace"] = c1_cold_kernel(
+            "scs_spmv_w64", pb, peak_gbs)
         # config 1 is the reference's plain CRS SpMV: both CRS schedules too
         extra["c1_poisson512_f64_crs"] = {
             "lane_per_row": c1_times(lambda: ctx.crs_spmv(capi.F64, P.n_rows, P.n_cols, P.nnz,
                                                           prp, pcol, pval, pdx, pdy, None)),
             "row_blocks": c1_times(lambda: ctx.crs_spmv(capi.F64, P.n_rows, P.n_cols, P.nnz,
                                                         prp, pcol, pval, pdx, pdy, pplan))}
+        for key, kern in (("lane_per_row", "crs_spmv_lane"), ("row_blocks", "crs_spmv_blocked")):
+            extra["c1_poisson512_f64_crs"][key]["kernel_trace"] = c1_cold_kernel(kern, pb, peak_gbs)
         # the dispatch floor: an 8-byte fill kernel timed the same two ways
         # (both C1 timings are within a few microseconds of it)
         floor = c1_times(lambda: ctx._check(ctx.lib.smax_hip_zero(ctx.handle, pdy.ptr, 8)))
@@ -695,6 +978,8 @@ def main() -> int:
     if want("cpu"):
         log(f"CPU baseline ({args.cpu_seconds} s sample)")
         base = cpu_baseline(A, x, args.cpu_seconds)
+        extra["cpu"] = cpu_configs(ctx, A, x, args.cpu_config_seconds)
+        base["host"] = extra["cpu"]["host"]
 
     if best is None:  # a partial run (--only without sell/crs): extras only
         print(json.dumps({"metric": METRIC, "value": None, "unit": "GB/s",
@@ -724,7 +1009,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"lomax_band N={A.n_rows} nnz={A.nnz} FP64 SpMV, "
                                f"best of CRS and SELL-C-sigma (C in {{32,64}}, "
-                               f"sigma in {{4096,N}}): {fmt}",
+                               f"sigma in {{1,64,4096,N}}): {fmt}",
                    "format": fmt, "n_rows": A.n_rows, "nnz": A.nnz,
                    "B_alg_bytes": balg},
         "roofline": roof,

@@ -221,19 +221,25 @@ void oracle_bcrs_spmm_f64(i64 n_rows, i64 b_h, i64 b_w, i64 h_pad, i64 w_pad,
 /*
  * Column-major: X[n_rows*v + col], Y[row + v*n_rows] (the reference indexes
  * X with n_rows as leading dimension, spmm_cpu_crs_impl.hpp:29); row-major:
- * X[col*k + v], Y[row*k + v].  Per output a sequential chain over j.
+ * X[col*k + v], Y[row*k + v].  The reference's loop nest
+ * (spmm_cpu_crs_impl.hpp:18-45, 58-87): per row a tmp[k] zeroed, then for
+ * each stored entry j a SIMD pass over the k columns, tmp[v] += val*X; so each
+ * output is a sequential chain over j (an fma under -march=native).
  */
 #define SPMM_BODY(T, LOAD, FMA)                                               \
     _Pragma("omp parallel for schedule(static)")                               \
     for (i64 r = 0; r < n_rows; ++r) {                                         \
+        T tmp[k];                                                              \
+        for (i64 v = 0; v < k; ++v) tmp[v] = 0;                                \
+        for (i32 j = row_ptr[r]; j < row_ptr[r + 1]; ++j) {                    \
+            const T a = LOAD(val[j]);                                          \
+            const T *xr = row_major ? X + (i64)col[j] * k : X + col[j];        \
+            const i64 xs = row_major ? 1 : n_rows;                             \
+            _Pragma("omp simd")                                                \
+            for (i64 v = 0; v < k; ++v) tmp[v] = FMA(a, xr[v * xs], tmp[v]);   \
+        }                                                                      \
         for (i64 v = 0; v < k; ++v) {                                          \
-            T acc = 0;                                                         \
-            for (i32 j = row_ptr[r]; j < row_ptr[r + 1]; ++j) {                \
-                i64 xi = row_major ? (i64)col[j] * k + v                       \
-                                   : n_rows * v + (i64)col[j];                 \
-                acc = FMA(LOAD(val[j]), X[xi], acc);                           \
-            }                                                                  \
-            if (row_major) Y[r * k + v] = acc; else Y[r + v * n_rows] = acc;   \
+            if (row_major) Y[r * k + v] = tmp[v]; else Y[r + v * n_rows] = tmp[v]; \
         }                                                                      \
     }
 

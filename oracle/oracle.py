@@ -74,18 +74,21 @@ _SUFFIX = {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32",
 _VEC = {"f64": np.float64, "f32": np.float32, "bf16": np.float32}
 
 
-def crs_spmv(row_ptr, col, val, x) -> np.ndarray:
+def crs_spmv(row_ptr, col, val, x, out=None) -> np.ndarray:
+    """y = A x; `out` (n_rows entries of the vector type) is written if given."""
     s = _SUFFIX[val.dtype]
     n = len(row_ptr) - 1
-    y = np.empty(n, _VEC[s])
+    y = np.empty(n, _VEC[s]) if out is None else out
+    assert y.dtype == _VEC[s] and y.size >= n
     getattr(_load(), f"oracle_crs_spmv_{s}")(n, _p(row_ptr), _p(col), _p(val),
                                              _p(x), _p(y))
     return y
 
 
-def scs_spmv(C_, n_chunks, chunk_ptr, chunk_len, col, val, x) -> np.ndarray:
+def scs_spmv(C_, n_chunks, chunk_ptr, chunk_len, col, val, x, out=None) -> np.ndarray:
     s = _SUFFIX[val.dtype]
-    y = np.empty(n_chunks * C_, _VEC[s])
+    y = np.empty(n_chunks * C_, _VEC[s]) if out is None else out
+    assert y.dtype == _VEC[s] and y.size >= n_chunks * C_
     getattr(_load(), f"oracle_scs_spmv_{s}")(C_, n_chunks, _p(chunk_ptr),
                                              _p(chunk_len), _p(col), _p(val),
                                              _p(x), _p(y))
@@ -104,19 +107,21 @@ def bcrs_spmv(nbr, b_h, b_w, h_pad, w_pad, col, row_ptr, val, col_major, x,
 
 
 def bcrs_spmm(nbr, b_h, b_w, h_pad, w_pad, col, row_ptr, val, col_major, k,
-              X) -> np.ndarray:
+              X, out=None) -> np.ndarray:
     s = _SUFFIX[val.dtype]
-    Y = np.zeros(nbr * h_pad * k, _VEC[s])
+    Y = np.zeros(nbr * h_pad * k, _VEC[s]) if out is None else out
+    assert Y.dtype == _VEC[s] and Y.size >= nbr * h_pad * k
     getattr(_load(), f"oracle_bcrs_spmm_{s}")(nbr, b_h, b_w, h_pad, w_pad,
                                               _p(col), _p(row_ptr), _p(val),
                                               int(col_major), k, _p(X), _p(Y))
     return Y
 
 
-def crs_spmm(row_ptr, col, val, k, row_major, X) -> np.ndarray:
+def crs_spmm(row_ptr, col, val, k, row_major, X, out=None) -> np.ndarray:
     s = _SUFFIX[val.dtype]
     n = len(row_ptr) - 1
-    Y = np.empty(n * k, _VEC[s])
+    Y = np.empty(n * k, _VEC[s]) if out is None else out
+    assert Y.dtype == _VEC[s] and Y.size >= n * k
     getattr(_load(), f"oracle_crs_spmm_{s}")(n, _p(row_ptr), _p(col), _p(val),
                                              k, int(row_major), _p(X), _p(Y))
     return Y

@@ -66,3 +66,70 @@ def padded_counts(row_ptr: np.ndarray, C: int) -> np.ndarray:
     c = np.zeros(npad, np.int32)
     c[:n] = np.diff(row_ptr)
     return c
+
+
+def fd_gradient_check(A, v32, Xf, Wf, k, gv, gx, n_samples=64, seed=2024) -> dict:
+    """FP64 central differences of L = sum(Y o W), Y = A X (row-major X, W
+    with k columns), for n_samples value entries and n_samples X entries,
+    against the value gradient gv (CRS order, from the SDDMM) and the X
+    gradient gx (n_cols x k, from A^T W).  L is linear in each entry, so the
+    central difference (h = 2^-10 max(1, |entry|)) is exact up to rounding;
+    it is evaluated on the rows the entry touches (every other term of L
+    cancels identically).  Metric per entry: |g - g_FD| / max(|g_FD|,
+    sum of |terms|)."""
+    rng = np.random.default_rng(seed)
+    X = np.asarray(Xf, np.float64).reshape(A.n_cols, k)
+    W = np.asarray(Wf, np.float64).reshape(A.n_rows, k)
+    v64 = np.asarray(v32, np.float64)
+    rows_of = np.repeat(np.arange(A.n_rows), np.diff(A.row_ptr))
+
+    def loss(vals, Xm, rows):
+        tot = 0.0
+        for i in rows:
+            s, e = A.row_ptr[i], A.row_ptr[i + 1]
+            tot += float((vals[s:e] @ Xm[A.col[s:e]]) @ W[i])
+        return tot
+
+    def central(arr, at, rows):
+        # perturb arr[at] by +-h in place (arr is v64 or X), then restore
+        keep = arr[at]
+        h = 2.0 ** -10 * max(1.0, abs(keep))
+        arr[at] = keep + h
+        lp = loss(v64, X, rows)
+        arr[at] = keep - h
+        lm = loss(v64, X, rows)
+        arr[at] = keep
+        return (lp - lm) / (2 * h)
+
+    worst_v = worst_x = 0.0
+    for j in rng.choice(A.nnz, n_samples, replace=False):
+        i = rows_of[j]
+        fd = central(v64, j, [i])
+        scale = max(abs(fd), float(np.abs(W[i] * X[A.col[j]]).sum()))
+        worst_v = max(worst_v, abs(float(gv[j]) - fd) / scale)
+    for idx in rng.choice(A.n_cols * k, n_samples, replace=False):
+        jr, vv = divmod(int(idx), k)
+        mask = A.col == jr
+        rows = np.unique(rows_of[mask])
+        fd = central(X, (jr, vv), rows)
+        scale = max(abs(fd), float(np.abs(v64[mask] * W[rows_of[mask], vv]).sum()))
+        worst_x = max(worst_x, abs(float(gx[jr, vv]) - fd) / scale)
+    return {"entries": 2 * n_samples, "max_rel_err": max(worst_v, worst_x),
+            "max_rel_err_values": worst_v, "max_rel_err_X": worst_x, "bound": 1e-6,
+            "metric": "|g - g_FD| / max(|g_FD|, sum |terms|), FP64 central differences"}
+
+
+def sddmm_tree(k: int) -> tuple[int, int]:
+    """The SDDMM reduction tree for k: g lanes per element, each summing a run
+    of q consecutive v, then a butterfly over the g lanes."""
+    p = 1
+    while p < k:
+        p *= 2
+    g = min(64, max(1, p // 4))
+    return g, -(-k // g)
+
+
+def gamma(n, u):
+    """gamma_n = n u / (1 - n u): the relative error bound of n roundings."""
+    n = np.asarray(n, np.float64)
+    return n * u / (1 - n * u)

@@ -8,7 +8,7 @@ import pytest
 
 from oracle import oracle
 from smaxkernels_amd import capi, matgen
-from tests.helpers import assert_bitwise
+from tests.helpers import assert_bitwise, fd_gradient_check, sddmm_tree
 
 pytestmark = pytest.mark.gpu
 
@@ -49,16 +49,6 @@ def test_transpose_refresh_follows_new_values(ctx):
     assert_bitwise(T.arrays(np.float64)["val"],
                    oracle.crs_to_scs(trp, tcol, tval, 64, 256)["val"])
     T.destroy()
-
-
-def sddmm_tree(k):
-    """The SDDMM reduction tree for k: g lanes per element, each summing a
-    run of q consecutive v, then a butterfly over the g lanes."""
-    p = 1
-    while p < k:
-        p *= 2
-    g = min(64, max(1, p // 4))
-    return g, -(-k // g)
 
 
 @pytest.mark.parametrize("vt,k", [(capi.F32, 32), (capi.F64, 32), (capi.F32, 8),
@@ -183,16 +173,6 @@ class SpMMOp:
                                             self.col, v, 4, 64, 4096)
 
 
-def loss_rows_f64(A, vals, X, W, rows):
-    """sum_v W[i,v] * (A X)[i,v] over the given rows, in FP64."""
-    tot = 0.0
-    for i in rows:
-        s, e = A.row_ptr[i], A.row_ptr[i + 1]
-        yi = vals[s:e].astype(np.float64) @ X[A.col[s:e]].astype(np.float64)
-        tot += float(yi @ W[i].astype(np.float64))
-    return tot
-
-
 def test_autograd_matches_central_differences(ctx):
     """C4 at reduced N: FP32 values, k = 32, loss = sum(Y o W).  64 value and
     64 X entries; metric |g - g_FD| / max(|g_FD|, sum_v |W_iv X_jv|) <= 1e-6.
@@ -223,31 +203,5 @@ def test_autograd_matches_central_differences(ctx):
     np.testing.assert_allclose(
         Yh, oracle.crs_spmm(A.row_ptr, A.col, A.val.astype(np.float32), k, True,
                             Xh.ravel()).reshape(N, k), rtol=0, atol=0)
-    rng = np.random.default_rng(2024)
-    v64 = A.val.astype(np.float32).astype(np.float64)
-    rows_of = np.repeat(np.arange(N), np.diff(A.row_ptr))
-    worst = 0.0
-    for j in rng.choice(A.nnz, 64, replace=False):
-        i = rows_of[j]
-        h = 2.0 ** -10 * max(1.0, abs(v64[j]))
-        vp, vm = v64.copy(), v64.copy()
-        vp[j] += h
-        vm[j] -= h
-        fd = (loss_rows_f64(A, vp, Xh, Wh, [i]) - loss_rows_f64(A, vm, Xh, Wh, [i])) / (2 * h)
-        scale = max(abs(fd), float(np.abs(Wh[i].astype(np.float64) *
-                                          Xh[A.col[j]].astype(np.float64)).sum()))
-        worst = max(worst, abs(gv[j] - fd) / scale)
-    cols = A.col
-    for idx in rng.choice(N * k, 64, replace=False):
-        jr, vv = divmod(int(idx), k)
-        rows = np.unique(rows_of[cols == jr])
-        h = 2.0 ** -10 * max(1.0, abs(float(Xh[jr, vv])))
-        Xp = Xh.astype(np.float64).copy()
-        Xm = Xp.copy()
-        Xp[jr, vv] += h
-        Xm[jr, vv] -= h
-        fd = (loss_rows_f64(A, v64, Xp, Wh, rows) - loss_rows_f64(A, v64, Xm, Wh, rows)) / (2 * h)
-        mask = cols == jr
-        scale = max(abs(fd), float(np.abs(v64[mask] * Wh[rows_of[mask], vv]).sum()))
-        worst = max(worst, abs(gx[jr, vv] - fd) / scale)
-    assert worst <= 1e-6, worst
+    fd = fd_gradient_check(A, A.val.astype(np.float32), Xh, Wh, k, gv, gx)
+    assert fd["max_rel_err"] <= 1e-6, fd

@@ -130,13 +130,15 @@ def test_conversion_heap_fallback_matches_libstdcxx(ctx, sort_pin, tmp_path):
 
 
 def test_full_size_tie_order_digests(ctx):
-    """N = 4M Lomax workload: perm digests written from libstdc++ std::sort."""
+    """N = 4M Lomax workload, C in {32, 64} and every sigma of the committed
+    digests (1, 17, 64, 4096, N): perm and chunk_lengths digests written from
+    libstdc++ std::sort, and n_elements."""
     fx = json.loads((GOLDEN / "sort_digests.json").read_text())
     A = matgen.lomax_band(4_000_000)
     rp, col, v = ctx.upload(A.row_ptr), ctx.upload(A.col), ctx.upload(A.val)
     done = 0
     for case in fx["cases"]:
-        if case["N"] != A.n_rows or case["sigma"] < 4096:
+        if case["N"] != A.n_rows:
             continue
         s = ctx.scs_from_crs(A.n_rows, A.n_cols, A.nnz, rp, col, v, 8,
                              case["C"], case["sigma"])
@@ -147,7 +149,7 @@ def test_full_size_tie_order_digests(ctx):
         assert hashlib.sha256(cl.tobytes()).hexdigest() == case["chunk_lengths_sha256"]
         s.destroy()
         done += 1
-    assert done == 4
+    assert done == 10
     for d in (rp, col, v):
         d.free()
 
