@@ -91,15 +91,16 @@ def build_matgen() -> Path:
 
 
 def build_interface() -> Path:
-    """The C++ SMAX::Interface with the HIP platform (src/), linked against
-    libsmax_hip.so."""
+    """The C++ SMAX::Interface with the CPU and HIP platforms (src/), linked
+    against libsmax_hip.so."""
     OUT.mkdir(exist_ok=True)
     lib = OUT / "libsmaxkernels_hip.so"
     tmp = OUT / "libsmaxkernels_hip.so.tmp"
     cxx = shutil.which("g++") or "c++"
     src = PKG / "src"
-    _run([cxx, "-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-shared",
-          f"-I{INCLUDE}", f"-I{src}", str(src / "interface.cpp"),
+    # the CPU platform's kernels: OpenMP, and hardware fma for their chains
+    _run([cxx, "-std=c++17", "-O2", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-Wall",
+          "-Wextra", "-shared", f"-I{INCLUDE}", f"-I{src}", str(src / "interface.cpp"),
           str(src / "error_handler.cpp"), "-o", str(tmp), f"-L{OUT}",
           "-lsmax_hip", f"-Wl,-rpath,{OUT}"])
     os.replace(tmp, lib)

@@ -28,8 +28,6 @@ int Interface::register_kernel(const std::string &name, KernelType kernel_type,
                                FloatType float_type) {
     if (platform == PlatformType::CUDA)
         ErrorHandler::fatal("Cannot register CUDA kernel. SMAX not built with CUDA enabled.");
-    if (platform == PlatformType::CPU)
-        ErrorHandler::fatal("Cannot register CPU kernel. This SMAX build provides the HIP platform.");
     auto ctx = std::make_unique<KernelContext>(kernel_type, platform, int_type, float_type);
     switch (kernel_type) {
     case KernelType::SPMV:

@@ -1,12 +1,14 @@
 // SpMM kernel front-end (src/kernels/spmm.hpp:11-106): CRS A times a block
 // of k vectors.  The reference runs SpMM on the CPU only ("Platform not
-// supported" otherwise, spmm.hpp:75-84); here the HIP platform runs it.
+// supported" otherwise, spmm.hpp:75-84); here the CPU and HIP platforms run
+// it, with the same results bit for bit.
 #pragma once
 
 #include <algorithm>
 #include <stdexcept>
 
 #include "../kernel.hpp"
+#include "spmv/spmv_cpu.hpp"
 #include "spmv/spmv_hip.hpp"
 
 namespace SMAX::KERNELS {
@@ -71,8 +73,9 @@ class SpMMKernel : public Kernel {
     }
 
     int initialize(ULL, ULL, ULL) override {
-        if (!hip()) return 1;
+        if (!platform_ok()) return 1;
         ScopedPhase phase(timers.get(), "initialize");
+        if (k_ctx->platform_type == PlatformType::CPU) return 0;  // host arrays in place
         if (SPMV::HIP::check_types(k_ctx.get()) || stream.open()) return 1;
         smax_hip_ctx c = stream.get();
         const CRSMatrix &A = *args.A.crs;
@@ -98,7 +101,7 @@ class SpMMKernel : public Kernel {
     }
 
     int apply(ULL, ULL X_offset, ULL Y_offset) override {
-        if (!hip()) return 1;
+        if (!platform_ok()) return 1;
         ScopedPhase phase(timers.get(), "apply");
         const CRSMatrix &A = *args.A.crs;
         const ULL k = args.X.n_cols;
@@ -113,7 +116,15 @@ class SpMMKernel : public Kernel {
         const ULL y_need = flags.vec_row_major ? A.n_rows * k : (k - 1) * ld + A.n_rows;
         if (k == 0 || X_offset + x_need > args.X.n_rows * args.X.n_cols ||
             Y_offset + y_need > args.Y.n_rows * args.Y.n_cols) {
-                ErrorHandler::warning("SpMM: X or Y is smaller than the product addresses");
+            ErrorHandler::warning("SpMM: X or Y is smaller than the product addresses");
+            return 1;
+        }
+        if (k_ctx->platform_type == PlatformType::CPU) {
+            KERNELS::CPU::SpMMApply f{A, static_cast<const char *>(args.X.val) + xb * X_offset,
+                                      static_cast<char *>(args.Y.val) + xb * Y_offset, k,
+                                      flags.vec_row_major};
+            if (KERNELS::CPU::dispatch(*k_ctx, f) == 0) return 0;
+            ErrorHandler::warning("[CPU] SpMM: no kernel for the registered types");
             return 1;
         }
         int st = smax_hip_crs_spmm(
@@ -127,8 +138,9 @@ class SpMMKernel : public Kernel {
     }
 
     int finalize(ULL, ULL, ULL) override {
-        if (!hip()) return 1;
+        if (!platform_ok()) return 1;
         ScopedPhase phase(timers.get(), "finalize");
+        if (k_ctx->platform_type == PlatformType::CPU) return 0;
         int rc = PLATFORM::HIP::report(stream.get(), smax_hip_sync(stream.get()),
                                        "synchronize");
         if (!rc)
@@ -138,8 +150,10 @@ class SpMMKernel : public Kernel {
     }
 
   private:
-    bool hip() const {
-        if (k_ctx->platform_type == PlatformType::HIP) return true;
+    bool platform_ok() const {
+        if (k_ctx->platform_type == PlatformType::HIP ||
+            k_ctx->platform_type == PlatformType::CPU)
+            return true;
         std::cerr << "Error: Platform not supported\n";
         return false;
     }

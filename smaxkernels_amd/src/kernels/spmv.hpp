@@ -1,12 +1,13 @@
 // SpMV kernel front-end: registration (CRS 6 args, Sell-C-sigma 13, BCRS 10),
-// flags and the platform switch of src/kernels/spmv.hpp:12-201, with the HIP
-// platform behind initialize / apply / finalize.
+// flags and the platform switch of src/kernels/spmv.hpp:12-201, with the CPU
+// and HIP platforms behind initialize / apply / finalize.
 #pragma once
 
 #include <stdexcept>
 
 #include "../kernel.hpp"
 #include "spmv/spmv_common.hpp"
+#include "spmv/spmv_cpu.hpp"
 #include "spmv/spmv_hip.hpp"
 
 namespace SMAX::KERNELS {
@@ -95,25 +96,52 @@ class SpMVKernel : public Kernel {
         return 0;
     }
 
+    // CPU: the registered host arrays are used in place (initialize and
+    // finalize have nothing to move); HIP: copy in, launch, copy out.
     int initialize(ULL, ULL, ULL) override {
-        if (!hip()) return 1;
-        return SPMV::HIP::initialize(timers.get(), k_ctx.get(), &args, &flags, *dev);
+        switch (k_ctx->platform_type) {
+        case PlatformType::CPU: {
+            ScopedPhase phase(timers.get(), "initialize");
+            return 0;
+        }
+        case PlatformType::HIP:
+            return SPMV::HIP::initialize(timers.get(), k_ctx.get(), &args, &flags, *dev);
+        default:
+            return unsupported_platform();
+        }
     }
     int apply(ULL, ULL x_offset, ULL y_offset) override {
-        if (!hip()) return 1;
-        return SPMV::HIP::apply(timers.get(), k_ctx.get(), &args, &flags, *dev,
-                                x_offset, y_offset);
+        switch (k_ctx->platform_type) {
+        case PlatformType::CPU: {
+            ScopedPhase phase(timers.get(), "apply");
+            if (CPU::spmv_apply(*k_ctx, args, flags, x_offset, y_offset) == 0) return 0;
+            ErrorHandler::warning("[CPU] SpMV: no kernel for the registered types");
+            return 1;
+        }
+        case PlatformType::HIP:
+            return SPMV::HIP::apply(timers.get(), k_ctx.get(), &args, &flags, *dev,
+                                    x_offset, y_offset);
+        default:
+            return unsupported_platform();
+        }
     }
     int finalize(ULL, ULL, ULL) override {
-        if (!hip()) return 1;
-        return SPMV::HIP::finalize(timers.get(), k_ctx.get(), &args, &flags, *dev);
+        switch (k_ctx->platform_type) {
+        case PlatformType::CPU: {
+            ScopedPhase phase(timers.get(), "finalize");
+            return 0;
+        }
+        case PlatformType::HIP:
+            return SPMV::HIP::finalize(timers.get(), k_ctx.get(), &args, &flags, *dev);
+        default:
+            return unsupported_platform();
+        }
     }
 
   private:
-    bool hip() const {
-        if (k_ctx->platform_type == PlatformType::HIP) return true;
+    static int unsupported_platform() {
         std::cerr << "Error: Platform not supported\n";
-        return false;
+        return 1;
     }
 };
 

@@ -1,10 +1,12 @@
-// Tests of the C++ SMAX::Interface with the HIP platform.
+// Tests of the C++ SMAX::Interface with the CPU and HIP platforms.
 //
 //   run_tests host   API behaviour that needs no GPU (registration, argument
-//                    checks, refused platforms and types)
+//                    checks, refused platforms and types), and the
+//                    reference's known answers through PlatformType::CPU
 //   run_tests gpu    the reference's SpMV / SELL / BCRS / SpMM known answers
 //                    (examples/tests/*.hpp) run through PlatformType::HIP,
-//                    and the Jacobi application loop against a host Jacobi
+//                    the Jacobi application loop against a host Jacobi, and
+//                    the CPU and HIP platforms against each other bit for bit
 //
 // Exit status 0 when every selected test passes.
 #include <cmath>
@@ -78,10 +80,137 @@ std::vector<double> k_val = {1.1, 1.2, 2.2, 3.1, 3.3};
 // ------------------------------------------------------------------- host
 TEST(host, refuses_platforms_not_built) {
     SMAX::Interface smax;
-    expect_throw([&] { smax.register_kernel("a", KernelType::SPMV); }, "CPU platform");
     expect_throw([&] { smax.register_kernel("b", KernelType::SPMV, PlatformType::CUDA); },
                  "CUDA platform");
+    // the reference's default platform (interface.hpp:33-36) and HIP
+    check(smax.register_kernel("a", KernelType::SPMV) == 0, "CPU (default)");
     check(smax.register_kernel("c", KernelType::SPMV, PlatformType::HIP) == 0, "HIP");
+}
+
+// ------------------------------------------------------- host: CPU platform
+TEST(host, cpu_spmv_crs_kat) {  // examples/tests/spmv_test.hpp, default platform
+    SMAX::Interface smax;
+    std::vector<double> x(3, 1.0), y(3), want = {2.3, 2.2, 6.4};
+    smax.register_kernel("spmv", KernelType::SPMV);
+    auto *k = smax.kernel("spmv");
+    k->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+    k->register_B(3, x.data());
+    k->register_C(3, y.data());
+    check(k->run() == 0, "run");
+    close(y.data(), want.data(), 3, "spmv_y");
+}
+
+template <typename IT> void cpu_crs_index_type(IntType it) {
+    SMAX::Interface smax;
+    std::vector<IT> col(k_col.begin(), k_col.end()), rp(k_rp.begin(), k_rp.end());
+    std::vector<double> x(3, 1.0), y(3), want = {2.3, 2.2, 6.4};
+    smax.register_kernel("k", KernelType::SPMV, PlatformType::CPU, it);
+    auto *k = smax.kernel("k");
+    k->register_A(3, 3, 5, col.data(), rp.data(), k_val.data());
+    k->register_B(3, x.data());
+    k->register_C(3, y.data());
+    check(k->run() == 0, "run");
+    close(y.data(), want.data(), 3, "index type");
+}
+
+TEST(host, cpu_every_index_type) {  // the CPU dispatch covers all six (spmv_cpu.hpp:38-96)
+    cpu_crs_index_type<int16_t>(IntType::INT16);
+    cpu_crs_index_type<int32_t>(IntType::INT32);
+    cpu_crs_index_type<int64_t>(IntType::INT64);
+    cpu_crs_index_type<uint16_t>(IntType::UINT16);
+    cpu_crs_index_type<uint32_t>(IntType::UINT32);
+    cpu_crs_index_type<uint64_t>(IntType::UINT64);
+}
+
+TEST(host, cpu_spmv_scs_kat) {
+    // spmv_scs_2_4_test_1.hpp: C = 2, sigma = 4: rows {0, 2} form chunk 0
+    // (2 entries each), row 1 and a padding slot chunk 1; y in SELL order
+    SMAX::Interface smax;
+    std::vector<int> chunk_ptr = {0, 4, 6}, chunk_len = {2, 1}, col = {0, 0, 1, 2, 1, 0},
+                     perm = {0, 2, 1};
+    std::vector<double> val = {1.1, 3.1, 1.2, 3.3, 2.2, 0.0};
+    std::vector<double> x(3, 1.0), y(4, -1.0), want = {2.3, 6.4, 2.2, 0.0};
+    smax.register_kernel("scs", KernelType::SPMV, PlatformType::CPU);
+    auto *k = smax.kernel("scs");
+    k->set_mat_scs(true);
+    k->register_A(2, 4, 3, 4, 3, 2, 6, 5, chunk_ptr.data(), chunk_len.data(), col.data(),
+                  val.data(), perm.data());
+    k->register_B(3, x.data());
+    k->register_C(4, y.data());
+    check(k->run() == 0, "run");
+    close(y.data(), want.data(), 4, "scs_y");
+}
+
+TEST(host, cpu_bcrs_kats) {  // spmv_bcrs_1_1_test_3.hpp, both block layouts
+    std::vector<int> col = {0, 1, 1}, rp = {0, 2, 3};
+    // block (ix, jx): column-major at jx*2 + ix, row-major at ix*2 + jx
+    std::vector<double> cm = {1.1, 0.0, 0.0, 2.2, 1.3, 0.0, 1.4, 0.0, 0.0, 0.0, 0.0, 4.4};
+    std::vector<double> rm = {1.1, 0.0, 0.0, 2.2, 1.3, 1.4, 0.0, 0.0, 0.0, 0.0, 0.0, 4.4};
+    for (bool col_major : {true, false}) {
+        SMAX::Interface smax;
+        std::vector<double> x(4, 1.0), y(4), want = {3.8, 2.2, 0.0, 4.4};
+        smax.register_kernel("b", KernelType::SPMV, PlatformType::CPU);
+        auto *k = smax.kernel("b");
+        k->set_mat_bcrs(true);
+        k->register_A(2, 2, 3, 2, 2, 2, 2, col.data(), rp.data(),
+                      col_major ? cm.data() : rm.data());
+        k->register_B(4, x.data());
+        k->register_C(4, y.data());
+        k->set_block_column_major(col_major);
+        check(k->run() == 0, "run");
+        close(y.data(), want.data(), 4, col_major ? "bcrs cm" : "bcrs rm");
+    }
+}
+
+TEST(host, cpu_spmm_kats) {  // spmm_test.hpp (column-major), spmm_rowmaj_test.hpp
+    for (bool row_major : {false, true}) {
+        SMAX::Interface smax;
+        const int k = 4;
+        std::vector<double> X(3 * k, 1.0), Y(3 * k);
+        std::vector<double> want = {2.3, 2.2, 6.4, 2.3, 2.2, 6.4, 2.3, 2.2, 6.4, 2.3, 2.2, 6.4};
+        if (row_major) want = {2.3, 2.3, 2.3, 2.3, 2.2, 2.2, 2.2, 2.2, 6.4, 6.4, 6.4, 6.4};
+        smax.register_kernel("spmm", KernelType::SPMM);
+        auto *kk = smax.kernel("spmm");
+        kk->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+        kk->register_B(3, k, X.data());
+        kk->register_C(3, k, Y.data());
+        if (row_major) kk->set_vec_row_major(true);
+        check(kk->run() == 0, "run");
+        close(Y.data(), want.data(), 3 * k, "spmm_Y");
+    }
+}
+
+TEST(host, cpu_swap_operands_and_offsets) {
+    // x_{k+1} = A x_k three times through swap_operands, then one product at
+    // element offsets 1 (spmv_cpu_core.hpp:62-63): the same fma chains as a
+    // host loop, bit for bit
+    SMAX::Interface smax;
+    std::vector<double> a(3, 1.0), b(3, 0.0), host(3, 1.0), tmp(3);
+    smax.register_kernel("it", KernelType::SPMV, PlatformType::CPU);
+    auto *k = smax.kernel("it");
+    k->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+    k->register_B(3, a.data());
+    k->register_C(3, b.data());
+    for (int it = 0; it < 3; ++it) {
+        check(k->run() == 0, "run");
+        for (int r = 0; r < 3; ++r) {
+            double s = 0;
+            for (int j = k_rp[r]; j < k_rp[r + 1]; ++j) s = std::fma(k_val[j], host[k_col[j]], s);
+            tmp[r] = s;
+        }
+        host = tmp;
+        k->swap_operands();
+    }
+    check(std::memcmp(b.data(), host.data(), sizeof(double) * 3) == 0, "swapped iterate");
+    std::vector<double> xl = {9.0, 1.0, 2.0, 3.0}, yl(4, 0.0);
+    smax.register_kernel("off", KernelType::SPMV);
+    auto *o = smax.kernel("off");
+    o->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+    o->register_B(4, xl.data());
+    o->register_C(4, yl.data());
+    check(o->initialize() == 0 && o->apply(0, 1, 1) == 0 && o->finalize() == 0, "offsets");
+    const double w1 = std::fma(1.2, 2.0, 1.1 * 1.0);
+    check(yl[0] == 0.0 && yl[1] == w1, "offset y");
 }
 
 TEST(host, missing_kernel_throws) {
@@ -424,6 +553,98 @@ TEST(gpu, jacobi_poisson_matches_host_iterates) {
           "iteration count " + std::to_string(iters) + " vs host " + std::to_string(host_iters));
     check(std::memcmp(x_old->data(), ho.data(), sizeof(double) * N) == 0,
           "GPU-driven iterate differs from the host Jacobi");
+}
+
+namespace {
+
+// A 3000-row band matrix with 1..40 entries per row (sorted, distinct
+// columns), values and x from a fixed LCG: the cross-platform check input.
+struct Band {
+    int n = 3000;
+    std::vector<int> rp{0}, col;
+    std::vector<double> val, x;
+    Band() {
+        uint64_t s = 12345;
+        auto next = [&] { s = s * 6364136223846793005ULL + 1442695040888963407ULL; return s >> 11; };
+        for (int r = 0; r < n; ++r) {
+            const int len = 1 + static_cast<int>(next() % 40);
+            std::vector<int> c;
+            while (static_cast<int>(c.size()) < len) {
+                const int j = std::min(n - 1, std::max(0, r - 200 + static_cast<int>(next() % 401)));
+                if (std::find(c.begin(), c.end(), j) == c.end()) c.push_back(j);
+            }
+            std::sort(c.begin(), c.end());
+            for (int j : c) {
+                col.push_back(j);
+                val.push_back(double(next()) / double(1ULL << 53) * 2 - 1);
+            }
+            rp.push_back(static_cast<int>(col.size()));
+        }
+        for (int i = 0; i < n; ++i) x.push_back(double(next()) / double(1ULL << 53) * 2 - 1);
+    }
+};
+
+template <typename V, typename A>
+std::vector<A> run_spmv(PlatformType p, FloatType ft, const Band &m, const std::vector<V> &val,
+                        const std::vector<A> &x, SMAX::SpMVType impl) {
+    SMAX::Interface smax;
+    std::vector<A> xx(x), y(m.n, A(-7));
+    smax.register_kernel("k", KernelType::SPMV, p, IntType::INT32, ft);
+    auto *k = smax.kernel("k");
+    k->set_kernel_implementation(impl);
+    k->register_A(m.n, m.n, static_cast<int>(m.col.size()), const_cast<int *>(m.col.data()),
+                  const_cast<int *>(m.rp.data()), const_cast<V *>(val.data()));
+    k->register_B(m.n, xx.data());
+    k->register_C(m.n, y.data());
+    check(k->run() == 0, "run");
+    return y;
+}
+
+} // namespace
+
+TEST(gpu, cpu_and_hip_platforms_agree_bitwise) {
+    // The same registered arrays through PlatformType::CPU and ::HIP: CRS
+    // SpMV (both HIP schedules) in FP64, FP32 and BF16 values, and CRS SpMM
+    // in both layouts, equal bit for bit.
+    const Band m;
+    std::vector<float> v32(m.val.begin(), m.val.end()), x32(m.x.begin(), m.x.end());
+    std::vector<uint16_t> vbf(m.val.size());
+    for (size_t i = 0; i < vbf.size(); ++i) {
+        uint32_t u;
+        std::memcpy(&u, &v32[i], 4);
+        vbf[i] = static_cast<uint16_t>(u >> 16);  // truncation: any bfloat16 pattern will do
+    }
+    for (auto impl : {SMAX::SpMVType::naive_thread_per_row, SMAX::SpMVType::naive_warp_group}) {
+        auto a = run_spmv(PlatformType::CPU, FloatType::FLOAT64, m, m.val, m.x, impl);
+        auto b = run_spmv(PlatformType::HIP, FloatType::FLOAT64, m, m.val, m.x, impl);
+        check(std::memcmp(a.data(), b.data(), sizeof(double) * m.n) == 0, "FP64 CRS");
+        auto c = run_spmv(PlatformType::CPU, FloatType::FLOAT32, m, v32, x32, impl);
+        auto d = run_spmv(PlatformType::HIP, FloatType::FLOAT32, m, v32, x32, impl);
+        check(std::memcmp(c.data(), d.data(), sizeof(float) * m.n) == 0, "FP32 CRS");
+        auto e = run_spmv(PlatformType::CPU, FloatType::BFLOAT16, m, vbf, x32, impl);
+        auto f = run_spmv(PlatformType::HIP, FloatType::BFLOAT16, m, vbf, x32, impl);
+        check(std::memcmp(e.data(), f.data(), sizeof(float) * m.n) == 0, "BF16 CRS");
+    }
+    for (bool row_major : {false, true}) {
+        const int k = 8;
+        std::vector<double> X(m.n * k), Y[2];
+        for (int i = 0; i < m.n * k; ++i) X[i] = m.x[i % m.n] * double(1 + i % 3);
+        for (int p = 0; p < 2; ++p) {
+            SMAX::Interface smax;
+            Y[p].assign(m.n * k, -7.0);
+            smax.register_kernel("mm", KernelType::SPMM, p ? PlatformType::HIP : PlatformType::CPU);
+            auto *kk = smax.kernel("mm");
+            kk->register_A(m.n, m.n, static_cast<int>(m.col.size()),
+                           const_cast<int *>(m.col.data()), const_cast<int *>(m.rp.data()),
+                           const_cast<double *>(m.val.data()));
+            kk->register_B(m.n, k, X.data());
+            kk->register_C(m.n, k, Y[p].data());
+            if (row_major) kk->set_vec_row_major(true);
+            check(kk->run() == 0, "spmm run");
+        }
+        check(std::memcmp(Y[0].data(), Y[1].data(), sizeof(double) * m.n * k) == 0,
+              row_major ? "SpMM row-major" : "SpMM column-major");
+    }
 }
 
 int main(int argc, char **argv) {
