@@ -147,7 +147,10 @@ __device__ __forceinline__ A group_reduce_scatter(A (&cur)[U], int l) {
     return s;
 }
 
-template <int VT, int K>
+// SEQ: the handle's src_index is consecutive within a slot, so a slot's
+// destinations are its first element's src_index plus j (one src load per
+// slot instead of one per element).
+template <int VT, int K, bool SEQ>
 __global__ __launch_bounds__(256) void scs_sddmm_rows(
     int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
     const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ slot_len,
@@ -181,6 +184,8 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
     if (r >= n_rows) return;
     const int len = slot_len[slot];
     const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
+    // SEQ: CRS position of the slot's element 0 (its row start)
+    const int64_t row0 = SEQ && len > 0 ? src[base] : 0;
     A g[Q];
     load_run<A, Q>(Gm + static_cast<int64_t>(r) * K + l * Q, g);
     int j = 0;
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
         for (int u = 0; u < U; ++u) c0[u] = col[base + u * C];
 #pragma unroll
         for (int n = 0; n < kNpl; ++n)
-            s0[n] = src[base + static_cast<int64_t>(n * GL + lane_elem) * C];
+            s0[n] = SEQ ? 0 : src[base + static_cast<int64_t>(n * GL + lane_elem) * C];
         for (;;) {
             const bool more = j + 2 * U <= len;
             int32_t c1[U], s1[kNpl];
@@ -206,7 +211,8 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
                     c1[u] = col[base + static_cast<int64_t>(j + U + u) * C];
 #pragma unroll
                 for (int n = 0; n < kNpl; ++n)
-                    s1[n] = src[base + static_cast<int64_t>(j + U + n * GL + lane_elem) * C];
+                    s1[n] = SEQ ? 0
+                                : src[base + static_cast<int64_t>(j + U + n * GL + lane_elem) * C];
             }
             A xv[U][Q];
 #pragma unroll
@@ -222,14 +228,15 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
             }
             if constexpr (GL >= U) {
                 const A sum = group_reduce_scatter<GL, U>(part, l);
-                if (stores) dvals[s0[0]] = sum;
+                if (stores) dvals[SEQ ? row0 + j + lane_elem : s0[0]] = sum;
             } else {
 #pragma unroll
                 for (int n = 0; n < kNpl; ++n) {
                     A piece[GL];
 #pragma unroll
                     for (int i = 0; i < GL; ++i) piece[i] = part[n * GL + i];
-                    dvals[s0[n]] = group_reduce_scatter<GL, GL>(piece, l);
+                    dvals[SEQ ? row0 + j + n * GL + lane_elem : s0[n]] =
+                        group_reduce_scatter<GL, GL>(piece, l);
                 }
             }
             j += U;
@@ -247,7 +254,7 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
 #pragma unroll
         for (int t = 0; t < Q; ++t) acc = T::fma_(g[t], xv[t], acc);
         acc = group_sum<GL>(acc);
-        if (l == 0) dvals[src[e]] = acc;
+        if (l == 0) dvals[SEQ ? row0 + j : src[e]] = acc;
     }
 }
 
@@ -260,12 +267,21 @@ static void launch_sddmm_rows(smax_hip_ctx ctx, smax_hip_scs A, const void *Gm,
     const int64_t cpb = ck <= 256 ? 256 / ck : 1;
     const int64_t bpc = ck <= 256 ? 1 : ceil_div(ck, 256);
     const int64_t blocks = cpb > 1 ? ceil_div(in.n_chunks, cpb) : in.n_chunks * bpc;
-    hipLaunchKernelGGL((scs_sddmm_rows<VT, K>), dim3(static_cast<unsigned>(blocks)),
-                       dim3(256), 0, ctx->stream, in.C, in.n_chunks, in.n_rows, cpb, bpc,
-                       in.chunk_ptr, in.slot_len, A->sched.order_lf, in.col, in.slot_row,
-                       in.src_index, static_cast<const typename T::vec_t *>(Gm),
-                       static_cast<const typename T::vec_t *>(X),
-                       static_cast<typename T::vec_t *>(dv));
+    // the SpMV's occupancy cap, as for the SpMM, when long chunks lead
+    const unsigned lds =
+        static_cast<unsigned>(A->sched.order_lf ? A->sched.lds_cap_bytes : 0);
+#define SMAX_SDDMM_ROWS(SEQ)                                                               \
+    hipLaunchKernelGGL((scs_sddmm_rows<VT, K, SEQ>), dim3(static_cast<unsigned>(blocks)), \
+                       dim3(256), lds, ctx->stream, in.C, in.n_chunks, in.n_rows, cpb, bpc, \
+                       in.chunk_ptr, in.slot_len, A->sched.order_lf, in.col, in.slot_row,  \
+                       in.src_index, static_cast<const typename T::vec_t *>(Gm),           \
+                       static_cast<const typename T::vec_t *>(X),                          \
+                       static_cast<typename T::vec_t *>(dv))
+    if (A->src_consecutive)
+        SMAX_SDDMM_ROWS(true);
+    else
+        SMAX_SDDMM_ROWS(false);
+#undef SMAX_SDDMM_ROWS
 }
 
 template <int VT>

@@ -44,6 +44,10 @@ struct smax_hip_scs_s {
     int16_t *col_tab = nullptr;
     int col_tab_state = 0;
     smax_hip_scs_sched sched;     // built with the conversion
+    // src_index of a slot's stored elements is consecutive (the conversion
+    // took values in CRS order, without a source map): element j of a slot
+    // whose row starts at CRS position p has src_index p + j.
+    bool src_consecutive = false;
     // Built on first use by the SDDMM: row of every stored element (-1 for
     // padding), so an element's row needs no search.
     int32_t *elem_row = nullptr;

@@ -678,6 +678,7 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
     SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
     smax_hip_scs scs = new smax_hip_scs_s();
     scs->info = info;
+    scs->src_consecutive = src_map == nullptr;
     guard.keep = true;
     rc = build_schedule(ctx, C, sigma, n, n_chunks, n_elements, nnz, info.chunk_lengths,
                         &scs->sched);

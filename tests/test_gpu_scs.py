@@ -331,3 +331,50 @@ def test_invalid_arguments_are_rejected(ctx):
     with pytest.raises(capi.SmaxHipError) as e:
         ctx.scs_spmv(9, 64, 1, rp, rp, col, v, v, v)
     assert e.value.status == capi.ERROR_UNSUPPORTED
+
+
+def few_long_rows(n, sigma, n_long, long_len, short_len, seed):
+    """CRS whose sigma-windows hold n_long[w % len(n_long)] rows of long_len
+    entries (at random places) among rows of short_len (columns sorted,
+    distinct, within a +-n/8 band): after the window sort each window's first
+    chunk has that many long lanes over short ones, the chunks the execution
+    order starts first."""
+    rng = np.random.default_rng(seed)
+    L = np.full(n, short_len, np.int64)
+    for k, w in enumerate(range(0, n, sigma)):
+        m = min(sigma, n - w)
+        L[w + rng.choice(m, min(n_long[k % len(n_long)], m), replace=False)] = long_len
+    rp = np.concatenate([[0], np.cumsum(L)]).astype(np.int32)
+    half = n // 8
+    cols = []
+    for i in range(n):
+        lo, hi = max(0, i - half), min(n, i + half + 1)
+        cols.append(lo + np.sort(rng.choice(hi - lo, L[i], replace=False)))
+    col = np.concatenate(cols).astype(np.int32)
+    val = rng.uniform(-1, 1, rp[-1])
+    return Csr(n, n, rp, col, val)
+
+
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("n,sigma,n_long", [(70000, 4096, (5,)), (40000, 2048, (3, 17, 16)),
+                                            (9000, 9000, (16,)), (30000, 30000, (1,))])
+def test_scs_apply_few_long_rows(ctx, vt, n, sigma, n_long):
+    """Layouts whose long chunks hold a few long lanes over short ones: the
+    long-first execution order (XCD-remapped for sigma <= n/16, global for
+    sigma = n) with slot lengths.  Both y orders equal the oracle bit for
+    bit."""
+    A = few_long_rows(n, sigma, n_long, 700, 6, seed=n + sigma)
+    val, x = values_for(vt, A)
+    s = gpu_scs(ctx, A, 64, sigma, val)
+    want = oracle.crs_spmv(A.row_ptr, A.col, val, x)
+    dx = ctx.upload(x)
+    dy = ctx.zeros(A.n_rows, x.dtype)
+    s.apply(vt, dx, dy, unpermuted=True)
+    assert_bitwise(ctx.download(dy), want, "unpermuted")
+    yp = ctx.zeros(s.info.n_rows_padded, x.dtype)
+    s.apply(vt, dx, yp)
+    perm = s.arrays(val.dtype)["perm"]
+    assert_bitwise(ctx.download(yp)[perm], want, "permuted")
+    for d in (dx, dy, yp):
+        d.free()
+    s.destroy()
