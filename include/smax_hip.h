@@ -176,6 +176,28 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                       int row_major, const void *X, int64_t ldx, void *Y,
                       int64_t ldy, smax_hip_crs_plan plan);
 
+/*
+ * smax_hip_crs_spmm with no allocation of any kind, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Same arguments, checks and results (bit for bit) as smax_hip_crs_spmm.
+ * Column-major products with k >= 2 stage X and Y through a fixed 32 MiB
+ * device array of the library (one per device, held until the process
+ * ends) instead of the context's workspace, in passes of as many columns as
+ * fit: (n_rows + n_cols) * sizeof(vector type) * columns <= 32 MiB - 256.
+ * A product whose single column does not fit (n_rows + n_cols > 4,194,272
+ * for F64, 8,388,544 for F32/BF16) is refused with
+ * SMAX_HIP_ERROR_INVALID_VALUE.  Every call on one device shares that array,
+ * whatever its context: calls that overlap in time on one device (other
+ * streams, other contexts, other threads) race and must be serialised by the
+ * caller.  Row-major products and k = 1 use no scratch.
+ */
+int smax_hip_crs_spmm_noalloc(smax_hip_ctx ctx, int value_type, int64_t n_rows,
+                              int64_t n_cols, int64_t nnz, const int32_t *row_ptr,
+                              const int32_t *col, const void *val, int64_t k,
+                              int row_major, const void *X, int64_t ldx, void *Y,
+                              int64_t ldy, smax_hip_crs_plan plan);
+
 /* ------------------------------------------------------------ SELL-C-sigma */
 /*
  * y[c*C + i] = sum_{j < chunk_lengths[c]} val[o] * x[col[o]],

@@ -92,6 +92,8 @@ _SIGS = {
     "smax_hip_crs_plan_destroy": ([vp, vp], C.c_int),
     "smax_hip_crs_spmm": ([vp, C.c_int, i64, i64, i64, vp, vp, vp, i64, C.c_int,
                            vp, i64, vp, i64, vp], C.c_int),
+    "smax_hip_crs_spmm_noalloc": ([vp, C.c_int, i64, i64, i64, vp, vp, vp, i64, C.c_int,
+                                   vp, i64, vp, i64, vp], C.c_int),
     "smax_hip_scs_spmv": ([vp, C.c_int, i64, i64, vp, vp, vp, vp, vp, vp],
                           C.c_int),
     "smax_hip_scs_spmv_unpermuted": ([vp, C.c_int, i64, i64, i64, vp, vp, vp,

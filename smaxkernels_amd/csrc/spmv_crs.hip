@@ -18,6 +18,7 @@
 //    band of neighbouring blocks is shared in one L2.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -401,6 +402,43 @@ __global__ __launch_bounds__(256) void k_relayout(int64_t rows, int64_t k,
 
 }  // namespace smaxhip
 
+namespace smaxhip {
+
+// Fixed staging area of smax_hip_crs_spmm_noalloc (row-major copies of a
+// column block of X and of Y): a file-scope device array, so that entry
+// point allocates nothing.  Each device holds its own copy for the life of
+// the process, shared by every call on that device.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for the current
+// device; the name itself is a host-side stand-in.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[kMaxDevices];
+
+// The scratch array's device address on ctx's device (current on entry),
+// looked up once per device.
+static int noalloc_scratch(smax_hip_ctx ctx, void **out) {
+    *out = nullptr;
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return fail(ctx, SMAX_HIP_ERROR_NO_DEVICE,
+                    "smax_hip_crs_spmm_noalloc: device %d beyond %d", ctx->device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[ctx->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                            "hipGetSymbolAddress(scratch)");
+        g_scratch[ctx->device] = p;
+    }
+    *out = g_scratch[ctx->device];
+    return SMAX_HIP_SUCCESS;
+}
+
+}  // namespace smaxhip
+
 extern "C" {
 
 int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
@@ -608,6 +646,78 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                                0, ctx->stream, n_rows, k, nullptr, row_ptr, col, v, xv,
                                ldx, yv, ldy);
         SMAX_HIP_LAUNCHED(ctx, "crs_spmm_kernel");
+        return SMAX_HIP_SUCCESS;
+    });
+    return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+int smax_hip_crs_spmm_noalloc(smax_hip_ctx ctx, int value_type, int64_t n_rows,
+                              int64_t n_cols, int64_t nnz, const int32_t *row_ptr,
+                              const int32_t *col, const void *val, int64_t k,
+                              int row_major, const void *X, int64_t ldx, void *Y,
+                              int64_t ldy, smax_hip_crs_plan plan) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    // row-major products and k = 1 stage nothing: the same kernels, no scratch
+    if (row_major || k == 1)
+        return smax_hip_crs_spmm(ctx, value_type, n_rows, n_cols, nnz, row_ptr, col, val, k,
+                                 row_major, X, ldx, Y, ldy, plan);
+    if (!valid_value_type(value_type))
+        return fail(ctx, SMAX_HIP_ERROR_UNSUPPORTED,
+                    "smax_hip_crs_spmm_noalloc: value type %d", value_type);
+    SMAX_HIP_REQUIRE(ctx, n_rows > 0 && n_cols > 0 && nnz >= 0 && k > 0,
+                     "smax_hip_crs_spmm_noalloc: bad shape");
+    SMAX_HIP_REQUIRE(ctx, row_ptr && X && Y && (nnz == 0 || (col && val)),
+                     "smax_hip_crs_spmm_noalloc: null pointer");
+    SMAX_HIP_REQUIRE(ctx, (ldx >= n_cols || ldx >= n_rows) && ldy >= n_rows,
+                     "smax_hip_crs_spmm_noalloc: column-major ld too small (need ldx >= "
+                     "min(n_rows, n_cols) and ldy >= n_rows)");
+    if (plan)
+        SMAX_HIP_REQUIRE(ctx, plan->n_rows == n_rows && plan->nnz == nnz,
+                         "smax_hip_crs_spmm_noalloc: plan built for another pattern");
+    SMAX_HIP_DISPATCH_VT(value_type, {
+        using A = typename vt<VT>::vec_t;
+        const int64_t s = static_cast<int64_t>(sizeof(A));
+        // columns per pass: the X block (rounded up to 256 bytes) and the Y
+        // block must fit the scratch together
+        const int64_t per_col = (n_cols + n_rows) * s;
+        const int64_t kb_max = (static_cast<int64_t>(kNoAllocScratchBytes) - 256) / per_col;
+        if (kb_max < 1)
+            return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                        "smax_hip_crs_spmm_noalloc: one column of X and Y takes %lld bytes, "
+                        "more than the %zu-byte scratch (n_rows + n_cols <= %lld for this "
+                        "type); use smax_hip_crs_spmm",
+                        static_cast<long long>(per_col), kNoAllocScratchBytes,
+                        static_cast<long long>((static_cast<int64_t>(kNoAllocScratchBytes) - 256) / s));
+        int64_t kb = std::min(k, kb_max);
+        if (kb < k && kb >= 16) {
+            // a power of two in [16, 64] takes the 16-byte-run kernel
+            int64_t p = 16;
+            while (p * 2 <= std::min<int64_t>(kb, 64)) p *= 2;
+            kb = p;
+        }
+        void *base = nullptr;
+        int rc = noalloc_scratch(ctx, &base);
+        if (rc) return rc;
+        A *Xr = static_cast<A *>(base);
+        const int64_t xbytes = (n_cols * kb * s + 255) / 256 * 256;
+        A *Yr = reinterpret_cast<A *>(static_cast<char *>(base) + xbytes);
+        for (int64_t v0 = 0; v0 < k; v0 += kb) {
+            const int64_t kc = std::min(kb, k - v0);
+            const unsigned gy = static_cast<unsigned>(ceil_div(kc, 32));
+            hipLaunchKernelGGL((k_relayout<A, true>),
+                               dim3(static_cast<unsigned>(ceil_div(n_cols, 32)), gy),
+                               dim3(256), 0, ctx->stream, n_cols, kc,
+                               static_cast<const A *>(X) + v0 * ldx, ldx, Xr);
+            SMAX_HIP_LAUNCHED(ctx, "k_relayout");
+            rc = smax_hip_crs_spmm(ctx, value_type, n_rows, n_cols, nnz, row_ptr, col, val,
+                                   kc, 1, Xr, kc, Yr, kc, plan);
+            if (rc) return rc;
+            hipLaunchKernelGGL((k_relayout<A, false>),
+                               dim3(static_cast<unsigned>(ceil_div(n_rows, 32)), gy),
+                               dim3(256), 0, ctx->stream, n_rows, kc, Yr, ldy,
+                               static_cast<A *>(Y) + v0 * ldy);
+            SMAX_HIP_LAUNCHED(ctx, "k_relayout");
+        }
         return SMAX_HIP_SUCCESS;
     });
     return SMAX_HIP_ERROR_UNSUPPORTED;
