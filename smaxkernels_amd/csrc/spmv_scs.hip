@@ -21,10 +21,13 @@ namespace smaxhip {
 
 constexpr int kScsThreads = 256;
 constexpr int kScsUnroll = 8;       // generic-C chains
-// C = 64 chains: 16-element groups, 8 for BF16 values (measured on the
-// 4M-row band matrix: 0.48 vs 0.52 ms at sigma = 4096; FP32 and FP64 are
-// fastest at 16)
-template <int VT> constexpr int kScsPipeUnroll = VT == SMAX_HIP_BF16 ? 8 : 16;
+// C = 64 chains, elements per group (measured on the 4M-row band matrix):
+// FP64 24 (0.697 / 0.641 ms at sigma = 4096 / 65536; 16: 0.707 / 0.659, with
+// 128 VGPRs against 90 -- four waves per SIMD instead of five, each with more
+// gathers in flight; 28 and 32 leave two waves and lose at sigma = 4096),
+// FP32 16 (24: 0.519 vs 0.493 ms), BF16 values 8 (16: 0.52 vs 0.48 ms).
+template <int VT>
+constexpr int kScsPipeUnroll = VT == SMAX_HIP_BF16 ? 8 : VT == SMAX_HIP_F32 ? 16 : 24;
 
 // One slot's chain in groups of U: the next group's column indices and
 // values are requested before this group's x gathers (software pipelining),
