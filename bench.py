@@ -258,7 +258,7 @@ def cpu_baseline(A, x, seconds) -> dict:
                       f"doubling loop after a warm-up call"}
 
 
-def cpu_configs(ctx, A, x, seconds) -> dict:
+def cpu_configs(ctx, A, x, seconds, host) -> dict:
     """The oracle's OpenMP kernels (reference loop orders) on every config,
     each timed by the doubling loop for >= `seconds`; GFLOP/s and GB/s on
     the same algorithmic bytes as the GPU lines.
@@ -272,7 +272,7 @@ def cpu_configs(ctx, A, x, seconds) -> dict:
           adjacency (the reference CPU path runs k BCRS SpMVs; the oracle's
           k-column loop is that, with the columns innermost)"""
     from oracle import oracle
-    out = {"host": host_description(), "threads": cpu_threads(), "kind": "port",
+    out = {"host": host, "threads": cpu_threads(), "kind": "port",
            "method": "warm-up call, then doubling iteration counts until one batch "
                      f"lasts >= {seconds} s (benchmarks_common.hpp:60-95)"}
 
@@ -830,6 +830,8 @@ def main() -> int:
         return 2
     only = set(filter(None, args.only.split(",")))
     want = (lambda s: not only or s in only)
+    # before any OpenMP runtime pins this thread
+    host = host_description()
     extra: dict = {}
     ctx = capi.Context(0)
 
@@ -997,8 +999,8 @@ def main() -> int:
     if want("cpu"):
         log(f"CPU baseline ({args.cpu_seconds} s sample)")
         base = cpu_baseline(A, x, args.cpu_seconds)
-        extra["cpu"] = cpu_configs(ctx, A, x, args.cpu_config_seconds)
-        base["host"] = extra["cpu"]["host"]
+        extra["cpu"] = cpu_configs(ctx, A, x, args.cpu_config_seconds, host)
+        base["host"] = host
 
     if best is None:  # a partial run (--only without sell/crs): extras only
         print(json.dumps({"metric": METRIC, "value": None, "unit": "GB/s",

@@ -99,6 +99,10 @@ class SpMVKernel : public Kernel {
     // CPU: the registered host arrays are used in place (initialize and
     // finalize have nothing to move); HIP: copy in, launch, copy out.
     int initialize(ULL, ULL, ULL) override {
+        if (!registered() || !SPMV::operands_fit(args, flags, 0, 0)) {
+            ErrorHandler::warning("SpMV: x or y is smaller than the product reads or writes");
+            return 1;
+        }
         switch (k_ctx->platform_type) {
         case PlatformType::CPU: {
             ScopedPhase phase(timers.get(), "initialize");
@@ -111,6 +115,10 @@ class SpMVKernel : public Kernel {
         }
     }
     int apply(ULL, ULL x_offset, ULL y_offset) override {
+        if (!registered() || !SPMV::operands_fit(args, flags, x_offset, y_offset)) {
+            ErrorHandler::warning("SpMV: x or y is smaller than the product reads or writes");
+            return 1;
+        }
         switch (k_ctx->platform_type) {
         case PlatformType::CPU: {
             ScopedPhase phase(timers.get(), "apply");
@@ -139,6 +147,12 @@ class SpMVKernel : public Kernel {
     }
 
   private:
+    // the matrix of the registered format, and both vectors, are registered
+    bool registered() const {
+        const bool a = flags.is_mat_scs ? bool(args.A.scs)
+                                        : flags.is_mat_bcrs ? bool(args.A.bcrs) : bool(args.A.crs);
+        return a && args.x.val && args.y.val;
+    }
     static int unsupported_platform() {
         std::cerr << "Error: Platform not supported\n";
         return 1;

@@ -17,4 +17,29 @@ struct Flags {
     bool is_block_column_major = false;
 };
 
+// Entries of x the product reads and of y it writes, per format
+// (spmv_cuda_core.cuh:34-110 sizes the device copies the same way): CRS
+// n_cols / n_rows, SELL n_cols / n_chunks * C, BCRS n_cols * w_pad /
+// n_rows * h_pad.
+inline void operand_extents(const Args &a, const Flags &f, ULL &x_need, ULL &y_need) {
+    if (f.is_mat_scs) {
+        x_need = a.A.scs->n_cols;
+        y_need = a.A.scs->n_chunks * a.A.scs->C;
+    } else if (f.is_mat_bcrs) {
+        x_need = a.A.bcrs->n_cols * a.A.bcrs->width_pad;
+        y_need = a.A.bcrs->n_rows * a.A.bcrs->height_pad;
+    } else {
+        x_need = a.A.crs->n_cols;
+        y_need = a.A.crs->n_rows;
+    }
+}
+
+// Whether the registered x and y hold what a product at these element
+// offsets reads and writes.
+inline bool operands_fit(const Args &a, const Flags &f, ULL x_off, ULL y_off) {
+    ULL x_need = 0, y_need = 0;
+    operand_extents(a, f, x_need, y_need);
+    return x_off + x_need <= a.x.n_rows && y_off + y_need <= a.y.n_rows;
+}
+
 } // namespace SMAX::KERNELS::SPMV

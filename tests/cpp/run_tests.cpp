@@ -87,6 +87,28 @@ TEST(host, refuses_platforms_not_built) {
     check(smax.register_kernel("c", KernelType::SPMV, PlatformType::HIP) == 0, "HIP");
 }
 
+TEST(host, short_operands_are_refused) {
+    // x or y smaller than the product reads / writes (n_cols / n_rows entries
+    // past the element offset): run() and apply() report failure on either
+    // platform before touching memory or the GPU
+    for (auto p : {PlatformType::CPU, PlatformType::HIP}) {
+        SMAX::Interface smax;
+        std::vector<double> x(3, 1.0), y(3, 0.0);
+        smax.register_kernel("k", KernelType::SPMV, p);
+        auto *k = smax.kernel("k");
+        k->register_A(3, 3, 5, k_col.data(), k_rp.data(), k_val.data());
+        k->register_B(2, x.data());
+        k->register_C(3, y.data());
+        check(k->run() == 1, "x of 2 entries for 3 columns");
+        k->register_B(3, x.data());
+        k->register_C(2, y.data());
+        check(k->run() == 1, "y of 2 entries for 3 rows");
+        k->register_C(3, y.data());
+        check(k->apply(0, 1, 0) == 1 && k->apply(0, 0, 1) == 1, "offsets past the end");
+        check(y[0] == 0.0 && y[1] == 0.0 && y[2] == 0.0, "y untouched");
+    }
+}
+
 // ------------------------------------------------------- host: CPU platform
 TEST(host, cpu_spmv_crs_kat) {  // examples/tests/spmv_test.hpp, default platform
     SMAX::Interface smax;

@@ -25,6 +25,22 @@ def test_stream_copy_and_triad(ctx, n):
         d.free()
 
 
+@pytest.mark.parametrize("loads", [1, 2, 4, 8])
+@pytest.mark.parametrize("n", [1, 2, 3, 1001, 65537, 3_000_001])
+def test_stream_copy_wide(ctx, loads, n):
+    """The copy with several 16-byte loads in flight per lane moves every
+    element, including the odd last one and partial workgroup spans; an
+    unsupported load count is refused."""
+    src = np.arange(n, dtype=np.float64) * 1.5 - 7.0
+    ds, dd = ctx.upload(src), ctx.upload(np.full(n, -1.0))
+    ctx.stream_copy_wide(ds, dd, n, loads)
+    assert np.array_equal(ctx.download(dd), src)
+    st = ctx.lib.smax_hip_stream_copy_wide(ctx.handle, ds.ptr, dd.ptr, n, 3)
+    assert st == capi.ERROR_INVALID_VALUE
+    for d in (ds, dd):
+        d.free()
+
+
 @pytest.mark.parametrize("per_lane", [8, 16, 32])
 @pytest.mark.parametrize("n,window", [(100_003, 1), (100_003, 1000), (100_003, 100_003),
                                       (4096, 4000)])
