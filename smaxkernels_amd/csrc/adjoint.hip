@@ -94,7 +94,8 @@ __global__ __launch_bounds__(256) void scs_sddmm_kernel(
 // registers for the whole row; each element costs one coalesced X row read
 // with 16-byte loads, Q fmas per lane and the butterfly over GL lanes.  The
 // chain over the row's elements is software-pipelined (the next U column and
-// src indices are loaded before this group's X rows), and the U results of a
+// src indices are loaded before this group's X rows; U = kSddmmChainGroup),
+// and the U results of a
 // group, consecutive CRS positions of one row, leave in one store
 // instruction from U lanes (U / GL instructions when GL < U).  The U
 // elements of a group share one butterfly (group_reduce_scatter; per GL
@@ -147,6 +148,14 @@ __device__ __forceinline__ A group_reduce_scatter(A (&cur)[U], int l) {
     return s;
 }
 
+// Elements per pipelined group of the row kernel, measured at N = 1M (band
+// matrix, SELL-64-65536): 16 for k = 16 and 32 (FP64 k = 16 0.61 -> 0.51 ms,
+// k = 32 0.80 -> 0.69 ms; FP32 k = 32 0.403 -> 0.385 ms), 8 for k = 64 (FP32
+// 0.71 ms against 0.77 with 16) and for k <= 8, where a lane holds several
+// elements of a group and 16 would leave three or four waves per SIMD.
+template <int K>
+constexpr int kSddmmChainGroup = K == 16 || K == 32 ? 16 : 8;
+
 // SEQ: the handle's src_index is consecutive within a slot, so a slot's
 // destinations are its first element's src_index plus j (one src load per
 // slot instead of one per element).
@@ -163,7 +172,7 @@ __global__ __launch_bounds__(256) void scs_sddmm_rows(
     using A = typename T::vec_t;
     constexpr int GL = static_cast<int>(sddmm_group(K));
     constexpr int Q = K / GL;
-    constexpr int U = 8;
+    constexpr int U = kSddmmChainGroup<K>;
     static_assert(GL * Q == K, "K must be a power of two");
     const int64_t ck = C * GL;
     int64_t pos, li;
