@@ -400,6 +400,98 @@ __global__ __launch_bounds__(256) void k_relayout(int64_t rows, int64_t k,
     }
 }
 
+// The same relayout with 16-byte accesses on both sides, for operands whose
+// unit-stride extents (rows and k) and leading dimension are multiples of
+// E = 16 / sizeof(T) and whose pointers are 16-byte aligned: a 64-row x
+// 32-column tile, the column-major side read or written as runs of E rows of
+// one column, the row-major side as runs of E columns of one row.  The LDS
+// tile is kept as [column][row] with a pitch of 65 elements, so the E-row
+// runs of the column-major side and the E-column gathers of the row-major
+// side both spread over the banks.
+template <typename T, bool COL_IN>
+__global__ __launch_bounds__(256) void k_relayout_vec(int64_t rows, int64_t k,
+                                                      const T *__restrict__ in,
+                                                      int64_t ld, T *__restrict__ out) {
+    constexpr int E = 16 / static_cast<int>(sizeof(T));
+    constexpr int TR = 64, TV = 32, P = TR + 1;
+    constexpr int VPC = TR / E;   // runs per tile column (column-major side)
+    constexpr int CPP = 256 / VPC;
+    constexpr int VPR = TV / E;   // runs per tile row (row-major side)
+    constexpr int RPP = 256 / VPR;
+    using V = T __attribute__((ext_vector_type(E)));
+    __shared__ T tile[TV * P];   // tile[(v - v0) * P + (r - r0)]
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * TR;
+    const int64_t v0 = static_cast<int64_t>(blockIdx.y) * TV;
+    const int t = static_cast<int>(threadIdx.x);
+    if (COL_IN) {
+#pragma unroll
+        for (int p = 0; p < TV; p += CPP) {
+            const int vc = p + t / VPC, rc = (t % VPC) * E;
+            const int64_t r = r0 + rc, v = v0 + vc;
+            if (r < rows && v < k) {
+                const V e = *reinterpret_cast<const V *>(in + r + v * ld);
+#pragma unroll
+                for (int i = 0; i < E; ++i) tile[vc * P + rc + i] = e[i];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < TR; p += RPP) {
+            const int rr = p + t / VPR, vv = (t % VPR) * E;
+            const int64_t r = r0 + rr, v = v0 + vv;
+            if (r < rows && v < k) {
+                V e;
+#pragma unroll
+                for (int i = 0; i < E; ++i) e[i] = tile[(vv + i) * P + rr];
+                *reinterpret_cast<V *>(out + r * k + v) = e;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < TR; p += RPP) {
+            const int rr = p + t / VPR, vv = (t % VPR) * E;
+            const int64_t r = r0 + rr, v = v0 + vv;
+            if (r < rows && v < k) {
+                const V e = *reinterpret_cast<const V *>(in + r * k + v);
+#pragma unroll
+                for (int i = 0; i < E; ++i) tile[(vv + i) * P + rr] = e[i];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < TV; p += CPP) {
+            const int vc = p + t / VPC, rc = (t % VPC) * E;
+            const int64_t r = r0 + rc, v = v0 + vc;
+            if (r < rows && v < k) {
+                V e;
+#pragma unroll
+                for (int i = 0; i < E; ++i) e[i] = tile[vc * P + rc + i];
+                *reinterpret_cast<V *>(out + r + v * ld) = e;
+            }
+        }
+    }
+}
+
+// Column-major <-> row-major copy of a rows x k block (COL_IN: column-major
+// `in` with leading dimension ld to row-major `out`; else the reverse), with
+// 16-byte accesses where the extents and alignment allow.
+template <typename T, bool COL_IN>
+static void launch_relayout(smax_hip_ctx ctx, int64_t rows, int64_t k, const T *in, int64_t ld,
+                            T *out) {
+    constexpr int64_t E = 16 / sizeof(T);
+    const bool vec = rows % E == 0 && k % E == 0 && ld % E == 0 &&
+                     ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const unsigned gy = static_cast<unsigned>(ceil_div(k, 32));
+    if (vec)
+        hipLaunchKernelGGL((k_relayout_vec<T, COL_IN>),
+                           dim3(static_cast<unsigned>(ceil_div(rows, 64)), gy), dim3(256), 0,
+                           ctx->stream, rows, k, in, ld, out);
+    else
+        hipLaunchKernelGGL((k_relayout<T, COL_IN>),
+                           dim3(static_cast<unsigned>(ceil_div(rows, 32)), gy), dim3(256), 0,
+                           ctx->stream, rows, k, in, ld, out);
+}
+
 }  // namespace smaxhip
 
 namespace smaxhip {
@@ -603,19 +695,12 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
             if (rc) return rc;
             A *Xr = static_cast<A *>(w);
             A *Yr = reinterpret_cast<A *>(static_cast<char *>(w) + xb);
-            const unsigned gy = static_cast<unsigned>(ceil_div(k, 32));
-            hipLaunchKernelGGL((k_relayout<A, true>),
-                               dim3(static_cast<unsigned>(ceil_div(n_cols, 32)), gy),
-                               dim3(256), 0, ctx->stream, n_cols, k,
-                               static_cast<const A *>(X), ldx, Xr);
+            launch_relayout<A, true>(ctx, n_cols, k, static_cast<const A *>(X), ldx, Xr);
             SMAX_HIP_LAUNCHED(ctx, "k_relayout");
             rc = smax_hip_crs_spmm(ctx, value_type, n_rows, n_cols, nnz, row_ptr, col, val,
                                    k, 1, Xr, k, Yr, k, plan);
             if (rc) return rc;
-            hipLaunchKernelGGL((k_relayout<A, false>),
-                               dim3(static_cast<unsigned>(ceil_div(n_rows, 32)), gy),
-                               dim3(256), 0, ctx->stream, n_rows, k, Yr, ldy,
-                               static_cast<A *>(Y));
+            launch_relayout<A, false>(ctx, n_rows, k, Yr, ldy, static_cast<A *>(Y));
             SMAX_HIP_LAUNCHED(ctx, "k_relayout");
             return SMAX_HIP_SUCCESS;
         });
@@ -703,19 +788,13 @@ int smax_hip_crs_spmm_noalloc(smax_hip_ctx ctx, int value_type, int64_t n_rows,
         A *Yr = reinterpret_cast<A *>(static_cast<char *>(base) + xbytes);
         for (int64_t v0 = 0; v0 < k; v0 += kb) {
             const int64_t kc = std::min(kb, k - v0);
-            const unsigned gy = static_cast<unsigned>(ceil_div(kc, 32));
-            hipLaunchKernelGGL((k_relayout<A, true>),
-                               dim3(static_cast<unsigned>(ceil_div(n_cols, 32)), gy),
-                               dim3(256), 0, ctx->stream, n_cols, kc,
-                               static_cast<const A *>(X) + v0 * ldx, ldx, Xr);
+            launch_relayout<A, true>(ctx, n_cols, kc, static_cast<const A *>(X) + v0 * ldx, ldx,
+                                     Xr);
             SMAX_HIP_LAUNCHED(ctx, "k_relayout");
             rc = smax_hip_crs_spmm(ctx, value_type, n_rows, n_cols, nnz, row_ptr, col, val,
                                    kc, 1, Xr, kc, Yr, kc, plan);
             if (rc) return rc;
-            hipLaunchKernelGGL((k_relayout<A, false>),
-                               dim3(static_cast<unsigned>(ceil_div(n_rows, 32)), gy),
-                               dim3(256), 0, ctx->stream, n_rows, kc, Yr, ldy,
-                               static_cast<A *>(Y) + v0 * ldy);
+            launch_relayout<A, false>(ctx, n_rows, kc, Yr, ldy, static_cast<A *>(Y) + v0 * ldy);
             SMAX_HIP_LAUNCHED(ctx, "k_relayout");
         }
         return SMAX_HIP_SUCCESS;

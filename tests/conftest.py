@@ -8,6 +8,15 @@ ROOT = Path(__file__).resolve().parents[1]
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
+# torch ships its own HIP runtime: imported before libsmax_hip.so is loaded,
+# the library binds to that runtime and both see the GPU; loaded after it, a
+# second runtime finds no device.  Import it first so the tests that share
+# streams with torch pass whichever files run.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line(

@@ -128,18 +128,21 @@ def test_crs_spmm_bit_exact(ctx, vt, k, row_major, planned):
 
 
 @pytest.mark.parametrize("vt", VTS)
-def test_crs_spmm_column_major_padded_and_growing(ctx, vt):
+@pytest.mark.parametrize("pad_x,pad_y", [(5, 3), (4, 8)])
+def test_crs_spmm_column_major_padded_and_growing(ctx, vt, pad_x, pad_y):
     """Column-major operands with leading dimensions past n_rows, widths off
     the 32-wide staging tile, and a workspace that grows (k = 64 after 37)
     and is then reused (k = 3): Y equals the reference column by column, and
-    the padding between columns is left untouched."""
+    the padding between columns is left untouched.  Odd pads stage through
+    the element-wise relayout; pads of 4 and 8 (with k = 64) through the
+    16-byte one, whose last row tile (9000 = 140 * 64 + 40) is partial."""
     A = matgen.lomax_band(9000, half=1000)
     n = A.n_rows
     rp, col = ctx.upload(A.row_ptr), ctx.upload(A.col)
     for k in (37, 64, 3):
         val, X = operands(vt, A, k)  # X: n x k column-major, ld n
         v = ctx.upload(val)
-        ldx, ldy = n + 5, n + 3
+        ldx, ldy = n + pad_x, n + pad_y
         Xp = np.zeros(ldx * k, X.dtype)
         for c in range(k):
             Xp[c * ldx:c * ldx + n] = X[c * n:(c + 1) * n]
