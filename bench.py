@@ -37,6 +37,10 @@ import sys
 import time
 from pathlib import Path
 
+# the CPUs this process may use, read before any OpenMP runtime loads (torch's
+# pins the main thread to one core when OMP_PROC_BIND is set)
+CPUS_AT_START = len(os.sched_getaffinity(0))
+
 import numpy as np
 
 ROOT = Path(__file__).resolve().parent
@@ -231,14 +235,14 @@ def host_description() -> dict:
     except (OSError, subprocess.SubprocessError) as e:
         info["lscpu_error"] = str(e)
     info["nproc"] = os.cpu_count()
-    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    info["affinity_cpus"] = CPUS_AT_START
     for var in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES"):
         info[var] = os.environ.get(var)
     return info
 
 
 def cpu_threads() -> int:
-    return int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return int(os.environ.get("OMP_NUM_THREADS", CPUS_AT_START))
 
 
 def cpu_baseline(A, x, seconds) -> dict:
@@ -830,7 +834,6 @@ def main() -> int:
         return 2
     only = set(filter(None, args.only.split(",")))
     want = (lambda s: not only or s in only)
-    # before any OpenMP runtime pins this thread
     host = host_description()
     extra: dict = {}
     ctx = capi.Context(0)

@@ -71,10 +71,13 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
 void scs_free_arrays(smax_hip_scs_info &info);
 
 // Builds the execution schedule of a SELL layout from its device chunk
-// lengths (synchronous); free_schedule releases it.
+// lengths (synchronous); free_schedule releases it.  slot_row (device, may be
+// null) gives each slot's original row, which lets layouts sorted over wide
+// windows run their chunks in row order.
 int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_padded,
                    int64_t n_chunks, int64_t n_elements, int64_t nnz,
-                   const int32_t *chunk_lengths, smax_hip_scs_sched *out);
+                   const int32_t *chunk_lengths, smax_hip_scs_sched *out,
+                   const int32_t *slot_row = nullptr);
 void free_schedule(smax_hip_scs_sched &s);
 
 }  // namespace smaxhip

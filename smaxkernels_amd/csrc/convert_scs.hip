@@ -681,7 +681,7 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
     scs->src_consecutive = src_map == nullptr;
     guard.keep = true;
     rc = build_schedule(ctx, C, sigma, n, n_chunks, n_elements, nnz, info.chunk_lengths,
-                        &scs->sched);
+                        &scs->sched, info.slot_row);
     if (rc) {
         smax_hip_scs_destroy(ctx, scs);
         return rc;

@@ -501,8 +501,13 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
 // Block order: windows much shorter than an XCD's share keep rows (and their
 // x bands) local, so each XCD gets a contiguous range of chunks (remap).  A
 // near-global sort puts the longest chunks first, where a contiguous split
-// would leave one XCD with all of them: the hardware's round-robin order is
-// kept then.
+// would leave one XCD with all of them, and runs each length's rows across
+// the whole matrix at once (every XCD then gathers from all of x).  When the
+// rows of the slots are known (a converted matrix), such layouts run their
+// chunks sorted by their first slot's row instead: the XCDs get contiguous
+// ranges of that sequence, so each holds one stretch of rows with its share
+// of every length, and rows in flight stay local as with short windows.
+// Without the rows, the hardware's round-robin order is kept.
 //
 // Chunk order: one wave runs a chunk's 64 rows; the longest chunks (rows of
 // up to thousands of entries) then set the kernel's critical path, and in
@@ -532,9 +537,12 @@ void free_schedule(smax_hip_scs_sched &s) {
 
 int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_padded,
                    int64_t n_chunks, int64_t n_elements, int64_t nnz,
-                   const int32_t *chunk_lengths, smax_hip_scs_sched *out) {
+                   const int32_t *chunk_lengths, smax_hip_scs_sched *out,
+                   const int32_t *slot_row) {
     *out = smax_hip_scs_sched{};
-    out->remap = sigma * 16 <= n_rows_padded;
+    const bool local = sigma * 16 <= n_rows_padded;
+    const bool by_row = !local && slot_row != nullptr && C <= kScsThreads;
+    out->remap = local || by_row;
     // Occupancy: with little padding every lane of a wave has gathers in
     // flight, and five waves per SIMD congest the path from L2 to the MALL;
     // reserving LDS for four workgroups per CU (four waves per SIMD) runs
@@ -554,16 +562,33 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
         std::max<int64_t>(kLongChunk, static_cast<int64_t>(std::ceil(8 * mean)));
     const int64_t n_long =
         std::count_if(cl.begin(), cl.end(), [&](int32_t l) { return l >= thr; });
-    if (n_long == 0) return SMAX_HIP_SUCCESS;
-    // positions [a, b) of `ord` receive chunks [a, min(b, n)): long chunks
-    // first (longest first) at every `spread`-th position when they fit,
-    // then the rest in chunk order
-    auto fill = [&](std::vector<int32_t> &ord, int64_t a, int64_t b, int64_t spread) {
+    if (n_long == 0 && !by_row) return SMAX_HIP_SUCCESS;
+    // by_row: the chunks sorted by the row of their first slot (stable)
+    std::vector<int32_t> seq;
+    if (by_row) {
+        std::vector<int32_t> first(static_cast<size_t>(n));
+        SMAX_HIP_CALL(ctx, hipMemcpy2DAsync(first.data(), sizeof(int32_t), slot_row,
+                                            sizeof(int32_t) * C, sizeof(int32_t), n,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+        SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+        seq.resize(static_cast<size_t>(n));
+        for (int64_t c = 0; c < n; ++c) seq[c] = static_cast<int32_t>(c);
+        std::stable_sort(seq.begin(), seq.end(),
+                         [&](int32_t p, int32_t q) { return first[p] < first[q]; });
+    }
+    // positions [a, b) of `ord` receive the chunks at [a, min(b, n)) of the
+    // chunk sequence (`sq`, or chunk order when null): long chunks first
+    // (longest first) at every `spread`-th position when they fit, then the
+    // rest in sequence order
+    auto fill = [&](std::vector<int32_t> &ord, int64_t a, int64_t b, int64_t spread,
+                    const std::vector<int32_t> *sq) {
         const int64_t e = std::min(b, n);
         if (e <= a) return;
         std::vector<int32_t> lg, rest;
-        for (int64_t c = a; c < e; ++c)
-            (cl[c] >= thr ? lg : rest).push_back(static_cast<int32_t>(c));
+        for (int64_t i = a; i < e; ++i) {
+            const int32_t c = sq ? (*sq)[i] : static_cast<int32_t>(i);
+            (cl[c] >= thr ? lg : rest).push_back(c);
+        }
         std::stable_sort(lg.begin(), lg.end(),
                          [&](int32_t p, int32_t q) { return cl[p] > cl[q]; });
         const int64_t m = static_cast<int64_t>(lg.size());
@@ -577,10 +602,14 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
         for (int64_t p = a; p < e; ++p)
             if (!taken[p - a]) ord[p] = rest[r++];
     };
-    std::vector<int32_t> lf(static_cast<size_t>(n), -1);
-    fill(lf, 0, n, 1);
-    int rc = upload_order(ctx, lf, &out->order_lf);
-    if (rc || C > kScsThreads) return rc;
+    if (n_long > 0) {
+        std::vector<int32_t> lf(static_cast<size_t>(n), -1);
+        fill(lf, 0, n, 1, nullptr);
+        const int rc = upload_order(ctx, lf, &out->order_lf);
+        if (rc) return rc;
+    }
+    if (C > kScsThreads) return SMAX_HIP_SUCCESS;
+    const std::vector<int32_t> *sq = by_row ? &seq : nullptr;
     const int64_t per_block = kScsThreads / C;  // chunks per workgroup
     const int64_t nb = ceil_div(n, per_block);
     std::vector<int32_t> ord(static_cast<size_t>(nb * per_block), -1);
@@ -590,10 +619,10 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
         for (int64_t k = 0; k < kXcd; ++k) {
             const int64_t s = k < rem ? k * (q + 1) : rem * (q + 1) + (k - rem) * q;
             const int64_t len = k < rem ? q + 1 : q;
-            fill(ord, s * per_block, (s + len) * per_block, per_block);
+            fill(ord, s * per_block, (s + len) * per_block, per_block, sq);
         }
     } else {
-        fill(ord, 0, nb * per_block, per_block);
+        fill(ord, 0, nb * per_block, per_block, sq);
     }
     return upload_order(ctx, ord, &out->order);
 }
