@@ -317,6 +317,19 @@ int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs);
 int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                        const void *x, void *y, int unpermuted);
 
+/*
+ * How smax_hip_scs_apply runs A (C = 64): the slots moved out of chunks
+ * that mix rows of very different lengths into tail waves of their own
+ * (n_slots; 0 when the layout has none), the number of those waves, and the
+ * entries they store (copies of those slots' elements, with padding).  A
+ * layout gets tail waves when its padding exceeds 3x its stored entries; in
+ * such a chunk the slots longer than its 8th-longest slot move when that
+ * slot is under half the chunk's length.  Results are unchanged bit for bit.
+ * No reference counterpart.
+ */
+int smax_hip_scs_get_tails(smax_hip_scs A, int64_t *n_slots, int64_t *n_waves,
+                           int64_t *n_entries);
+
 /* Re-gathers the SELL values from a CRS value array of the same pattern
  * (val_sell[e] = crs_val[src_index[e]], 0 for padding).  Asynchronous, except
  * when a solver left rounded value copies on the handle: those are dropped

@@ -109,6 +109,7 @@ _SIGS = {
     "smax_hip_scs_get_info": ([vp, C.POINTER(ScsInfo)], C.c_int),
     "smax_hip_scs_destroy": ([vp, vp], C.c_int),
     "smax_hip_scs_refresh_values": ([vp, vp, vp], C.c_int),
+    "smax_hip_scs_get_tails": ([vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)], C.c_int),
     "smax_hip_scs_apply": ([vp, C.c_int, vp, vp, vp, C.c_int], C.c_int),
     "smax_hip_bcrs_spmv": ([vp, C.c_int, i64, i64, i64, i64, i64, vp, vp, vp,
                             C.c_int, vp, vp], C.c_int),
@@ -453,6 +454,12 @@ class Scs:
         c = self.ctx
         c._check(c.lib.smax_hip_scs_sddmm(c.handle, vtype, self.handle, k,
                                           _ptr(G), _ptr(X), _ptr(dvals)))
+
+    def tails(self) -> tuple:
+        """(slots, waves, stored entries) of the handle's tail waves."""
+        out = [i64(), i64(), i64()]
+        self.ctx._check(self.ctx.lib.smax_hip_scs_get_tails(self.handle, *map(C.byref, out)))
+        return tuple(v.value for v in out)
 
     def refresh_values(self, crs_val) -> None:
         c = self.ctx

@@ -17,6 +17,27 @@ struct smax_hip_scs_sched {
     int64_t lds_cap_bytes = 0;
 };
 
+// Tail waves of a C = 64 layout whose chunks mix rows of very different
+// lengths (built with the conversion when padding exceeds 3x the stored
+// entries).  In a chunk whose 8th-longest slot is under half the chunk's
+// length, the slots longer than that 8th-longest one are taken out of the
+// chunk's wave and repacked, longest first, into waves of their own: a SELL-64
+// of their own whose element j of lane l of wave w sits at ptr[w] + j*64 + l,
+// a wave taking slots while it has fewer than 64 and their lengths sum to at
+// most kTailWaveWork.  The chunk's wave then runs only as long as its 8th-
+// longest slot.  Each slot's chain is still one lane's sequential chain over
+// the same elements in the same order.
+struct smax_hip_scs_tails {
+    int32_t *head = nullptr;  // per chunk: longer slots run in tail waves
+    int32_t *slot = nullptr;  // per tail-wave lane (n_blocks * 256): slot, -1 none
+    int64_t *ptr = nullptr;   // per tail wave: its first element
+    int32_t *col = nullptr;   // the tail waves' elements
+    void *val = nullptr;
+    int64_t n_blocks = 0;     // workgroups of tail waves (a multiple of 8; 0: none)
+    int64_t n_elements = 0;   // storage of col / val
+    int64_t n_slots = 0, n_waves = 0;
+};
+
 struct smax_hip_scs_s {
     smax_hip_scs_info info;
     // Built on first use by the CG solvers, freed with the handle:
@@ -51,6 +72,7 @@ struct smax_hip_scs_s {
     // Built on first use by the SDDMM: row of every stored element (-1 for
     // padding), so an element's row needs no search.
     int32_t *elem_row = nullptr;
+    smax_hip_scs_tails tails;     // built with the conversion (C = 64 only)
 };
 
 struct smax_hip_scs_plan_s {
@@ -79,5 +101,11 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
                    const int32_t *chunk_lengths, smax_hip_scs_sched *out,
                    const int32_t *slot_row = nullptr);
 void free_schedule(smax_hip_scs_sched &s);
+
+// Builds A's tail waves when its layout qualifies (synchronous); copies the
+// values into them again after a value refresh (asynchronous); frees them.
+int build_tails(smax_hip_ctx ctx, smax_hip_scs A);
+int refresh_tail_values(smax_hip_ctx ctx, smax_hip_scs A);
+void free_tails(smax_hip_scs_tails &t);
 
 }  // namespace smaxhip

@@ -682,6 +682,7 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
     guard.keep = true;
     rc = build_schedule(ctx, C, sigma, n, n_chunks, n_elements, nnz, info.chunk_lengths,
                         &scs->sched, info.slot_row);
+    if (!rc) rc = build_tails(ctx, scs);
     if (rc) {
         smax_hip_scs_destroy(ctx, scs);
         return rc;
@@ -730,6 +731,7 @@ int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs) {
     dev_free(scs->val_bf16);
     dev_free(scs->diag_f32);
     free_schedule(scs->sched);
+    free_tails(scs->tails);
     dev_free(scs->elem_row);
     delete scs;
     return SMAX_HIP_SUCCESS;
@@ -771,7 +773,8 @@ int smax_hip_scs_refresh_values(smax_hip_ctx ctx, smax_hip_scs scs,
                static_cast<uint16_t *>(in.val));
         break;
     }
-    return SMAX_HIP_SUCCESS;
+    // the tail waves hold copies of their slots' values
+    return refresh_tail_values(ctx, scs);
 }
 
 }  // extern "C"

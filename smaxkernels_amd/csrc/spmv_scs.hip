@@ -12,6 +12,7 @@
 // U dependent fmas, so each lane keeps 2U loads in flight.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -96,6 +97,12 @@ slot_chain_pipe(const int32_t *__restrict__ col,
 //          times (f(a) = fma(0, x0, a) satisfies f(f(a)) = f(a) for every a,
 //          signed zeros and non-finite x0 included), so results stay
 //          bit-identical to the full-chunk chain.
+//   tails (LEN only; t_slot null: none): the first t_blocks workgroups run
+//          the handle's tail waves (smax_hip_scs_tails), the rest the chunks,
+//          whose lanes longer than t_head[chunk] then leave at once.  Both
+//          kinds of wave run the same chain code on their own arrays;
+//          t_blocks is a multiple of 8, so the chunks' workgroups keep their
+//          XCD alignment.
 template <int VT, bool UNPERM, bool REMAP, bool LEN>
 __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
     int64_t n_chunks, int64_t n_rows, const int32_t *__restrict__ chunk_ptr,
@@ -104,18 +111,44 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
     const typename vt<VT>::val_t *__restrict__ val,
     const int32_t *__restrict__ slot_row,
     const typename vt<VT>::vec_t *__restrict__ x,
-    typename vt<VT>::vec_t *__restrict__ y) {
+    typename vt<VT>::vec_t *__restrict__ y, const int32_t *__restrict__ t_head,
+    const int32_t *__restrict__ t_slot, const int64_t *__restrict__ t_ptr,
+    const int32_t *__restrict__ t_col, const typename vt<VT>::val_t *__restrict__ t_val,
+    int64_t t_blocks) {
     using T = vt<VT>;
-    const int64_t blk = REMAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int64_t pos = blk * (kScsThreads / kWave) + (threadIdx.x / kWave);
-    const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
-    if (chunk < 0 || chunk >= n_chunks) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t slot = chunk * kWave + lane;
-    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + lane;
-    const int clen = chunk_len[chunk];
-    const int len = LEN ? slot_len[slot] : clen;
-    auto acc = slot_chain_pipe<VT, kScsPipeUnroll<VT>>(col, val, x, base, kWave, len);
+    const bool tails = LEN && t_slot != nullptr;
+    int64_t slot, base;
+    int clen, len;
+    const int32_t *cp;
+    const typename T::val_t *vp;
+    if (tails && static_cast<int64_t>(blockIdx.x) < t_blocks) {
+        const int64_t w = static_cast<int64_t>(blockIdx.x) * (kScsThreads / kWave) +
+                          threadIdx.x / kWave;
+        const int32_t ts = t_slot[w * kWave + lane];
+        if (ts < 0) return;
+        slot = ts;
+        base = t_ptr[w] + lane;
+        clen = chunk_len[ts / kWave];
+        len = slot_len[ts];
+        cp = t_col;
+        vp = t_val;
+    } else {
+        const int64_t hb = tails ? blockIdx.x - t_blocks : blockIdx.x;
+        const int64_t hn = tails ? gridDim.x - t_blocks : gridDim.x;
+        const int64_t blk = REMAP ? xcd_block(hb, hn) : hb;
+        const int64_t pos = blk * (kScsThreads / kWave) + (threadIdx.x / kWave);
+        const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
+        if (chunk < 0 || chunk >= n_chunks) return;
+        slot = chunk * kWave + lane;
+        base = static_cast<int64_t>(chunk_ptr[chunk]) + lane;
+        clen = chunk_len[chunk];
+        len = LEN ? slot_len[slot] : clen;
+        if (tails && len > t_head[chunk]) return;
+        cp = col;
+        vp = val;
+    }
+    auto acc = slot_chain_pipe<VT, kScsPipeUnroll<VT>>(cp, vp, x, base, kWave, len);
     if (LEN && len < clen) acc = T::fma_(T::widen(typename T::val_t(0)), x[0], acc);
     if (UNPERM) {
         const int32_t r = slot_row[slot];
@@ -420,18 +453,23 @@ struct W64Args {
     const int32_t *slot_row;
     const void *x;
     void *y;
+    const smax_hip_scs_tails *tails;  // nullptr or n_blocks == 0: none
 };
 
 template <int VT, bool UNPERM, bool REMAP, bool LEN>
 static void launch_w64(smax_hip_ctx ctx, const W64Args &a) {
     using T = vt<VT>;
+    const smax_hip_scs_tails *t = LEN && a.tails && a.tails->n_blocks > 0 ? a.tails : nullptr;
+    const int64_t tb = t ? t->n_blocks : 0;
     hipLaunchKernelGGL((scs_spmv_w64<VT, UNPERM, REMAP, LEN>),
-                       grid_for(a.n_chunks, kScsThreads / kWave), dim3(kScsThreads),
-                       static_cast<unsigned>(a.lds_bytes), ctx->stream, a.n_chunks, a.n_rows, a.chunk_ptr, a.chunk_len,
-                       a.slot_len, a.order, a.col,
+                       dim3(static_cast<unsigned>(ceil_div(a.n_chunks, kScsThreads / kWave) + tb)),
+                       dim3(kScsThreads), static_cast<unsigned>(a.lds_bytes), ctx->stream,
+                       a.n_chunks, a.n_rows, a.chunk_ptr, a.chunk_len, a.slot_len, a.order, a.col,
                        static_cast<const typename T::val_t *>(a.val), a.slot_row,
                        static_cast<const typename T::vec_t *>(a.x),
-                       static_cast<typename T::vec_t *>(a.y));
+                       static_cast<typename T::vec_t *>(a.y), t ? t->head : nullptr,
+                       t ? t->slot : nullptr, t ? t->ptr : nullptr, t ? t->col : nullptr,
+                       static_cast<const typename T::val_t *>(t ? t->val : nullptr), tb);
 }
 
 template <int VT, bool UNPERM>
@@ -449,7 +487,8 @@ static void launch_w64_sched(smax_hip_ctx ctx, bool remap, const W64Args &a) {
 
 // remap: XCD-contiguous order (right when sigma-windows keep rows local);
 // slot_len: optional per-slot lengths (nullptr: full chunk chains);
-// order: optional execution order for C = 64 (nullptr: chunk order).
+// order: optional execution order for C = 64 (nullptr: chunk order);
+// tails: the handle's tail waves (C = 64 with slot_len; nullptr: none).
 template <int VT>
 int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
                     int64_t n_rows, const int32_t *chunk_ptr,
@@ -457,11 +496,12 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
                     const void *val, const int32_t *slot_row, const void *x,
                     void *y, bool unperm, bool remap = true,
                     const int32_t *slot_len = nullptr,
-                    const int32_t *order = nullptr, int64_t lds_bytes = 0) {
+                    const int32_t *order = nullptr, int64_t lds_bytes = 0,
+                    const smax_hip_scs_tails *tails = nullptr) {
     using T = vt<VT>;
     if (C == kWave) {
         const W64Args a{lds_bytes, n_chunks, n_rows, chunk_ptr, chunk_len, slot_len,
-                        order, col, val, slot_row, x, y};
+                        order, col, val, slot_row, x, y, tails};
         if (unperm)
             launch_w64_sched<VT, true>(ctx, remap, a);
         else
@@ -627,6 +667,167 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
     return upload_order(ctx, ord, &out->order);
 }
 
+// ------------------------------------------------------------ tail waves
+// Layouts whose padding exceeds 3x their stored entries get tail waves (see
+// smax_hip_scs_tails).  Measured on the 4M-row band matrix (FP64): SELL-64-1
+// 2.32 -> 1.10 ms, SELL-64-64 2.09 -> 1.04 ms.  With less padding the long
+// slots are few, their waves carry the kernel's longest chains together and
+// finish last: SELL-64-4096 (2.5x) took 0.76 ms against 0.70 without.
+constexpr int kTailRank = 8;              // a chunk's head runs to its 8th-longest slot
+constexpr int64_t kTailWaveWork = 32768;  // elements per tail wave at most
+
+// Copies the elements of every tail slot into its wave's arrays: element j of
+// the slot (chunk c, lane i) at chunk_ptr[c] + j*64 + i goes to
+// t_ptr[w] + j*64 + l for its tail-wave lane l of wave w.  One thread per
+// tail-wave lane; t_col null: values only.
+template <typename E>
+__global__ void k_tail_fill(int64_t n_lanes, const int32_t *__restrict__ t_slot,
+                            const int64_t *__restrict__ t_ptr,
+                            const int32_t *__restrict__ slot_len,
+                            const int32_t *__restrict__ chunk_ptr,
+                            const int32_t *__restrict__ col, const E *__restrict__ val,
+                            int32_t *__restrict__ t_col, E *__restrict__ t_val) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n_lanes) return;
+    const int32_t ts = t_slot[p];
+    if (ts < 0) return;
+    const int64_t src = static_cast<int64_t>(chunk_ptr[ts / kWave]) + ts % kWave;
+    const int64_t dst = t_ptr[p / kWave] + p % kWave;
+    const int len = slot_len[ts];
+    for (int j = 0; j < len; ++j) {
+        const int64_t o = static_cast<int64_t>(j) * kWave;
+        if (t_col) t_col[dst + o] = col[src + o];
+        t_val[dst + o] = val[src + o];
+    }
+}
+
+static int tail_fill(smax_hip_ctx ctx, smax_hip_scs A, bool with_col) {
+    const smax_hip_scs_info &in = A->info;
+    const smax_hip_scs_tails &t = A->tails;
+    const int64_t lanes = t.n_blocks * kScsThreads;
+    int32_t *tc = with_col ? t.col : nullptr;
+#define SMAX_TAIL_FILL(E)                                                                 \
+    hipLaunchKernelGGL((k_tail_fill<E>), grid_for(lanes, kScsThreads), dim3(kScsThreads), \
+                       0, ctx->stream, lanes, t.slot, t.ptr, in.slot_len, in.chunk_ptr,   \
+                       in.col, static_cast<const E *>(in.val), tc, static_cast<E *>(t.val))
+    switch (in.val_bytes) {
+    case 8: SMAX_TAIL_FILL(uint64_t); break;
+    case 4: SMAX_TAIL_FILL(uint32_t); break;
+    default: SMAX_TAIL_FILL(uint16_t); break;
+    }
+#undef SMAX_TAIL_FILL
+    SMAX_HIP_LAUNCHED(ctx, "k_tail_fill");
+    return SMAX_HIP_SUCCESS;
+}
+
+void free_tails(smax_hip_scs_tails &t) {
+    dev_free(t.head);
+    dev_free(t.slot);
+    dev_free(t.ptr);
+    dev_free(t.col);
+    dev_free(t.val);
+    t = smax_hip_scs_tails{};
+}
+
+int build_tails(smax_hip_ctx ctx, smax_hip_scs A) {
+    const smax_hip_scs_info &in = A->info;
+    if (in.C != kWave || in.nnz <= 0 || in.n_elements <= 4 * in.nnz) return SMAX_HIP_SUCCESS;
+    const int64_t nc = in.n_chunks;
+    std::vector<int32_t> sl(static_cast<size_t>(nc * kWave)), cl(static_cast<size_t>(nc));
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(sl.data(), in.slot_len, sizeof(int32_t) * sl.size(),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(cl.data(), in.chunk_lengths, sizeof(int32_t) * nc,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    // per chunk: the head length, and the slots beyond it
+    std::vector<int32_t> head(static_cast<size_t>(nc));
+    std::vector<std::pair<int32_t, int32_t>> tail;  // (length, slot)
+    int32_t lens[kWave];
+    for (int64_t c = 0; c < nc; ++c) {
+        const int32_t *l = sl.data() + c * kWave;
+        std::copy(l, l + kWave, lens);
+        std::nth_element(lens, lens + kTailRank - 1, lens + kWave, std::greater<int32_t>());
+        const int32_t h = lens[kTailRank - 1];
+        head[c] = 2 * static_cast<int64_t>(h) < cl[c] ? h : cl[c];
+        for (int i = 0; i < kWave; ++i)
+            if (l[i] > head[c]) tail.emplace_back(l[i], static_cast<int32_t>(c * kWave + i));
+    }
+    if (tail.empty()) return SMAX_HIP_SUCCESS;
+    // longest first (ties in slot order), packed greedily into waves
+    std::sort(tail.begin(), tail.end(), [](const auto &a, const auto &b) {
+        return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    std::vector<int64_t> wave_len;  // each wave's longest slot (its first)
+    std::vector<int64_t> pos(tail.size());
+    int64_t cnt = kWave, work = 0;
+    for (size_t t = 0; t < tail.size(); ++t) {
+        const int64_t l = tail[t].first;
+        if (cnt == kWave || work + l > kTailWaveWork) {
+            wave_len.push_back(l);
+            cnt = work = 0;
+        }
+        pos[t] = static_cast<int64_t>(wave_len.size() - 1) * kWave + cnt;
+        ++cnt;
+        work += l;
+    }
+    const int64_t n_waves = static_cast<int64_t>(wave_len.size());
+    const int64_t per_block = kScsThreads / kWave;
+    const int64_t n_blocks = ceil_div(ceil_div(n_waves, per_block), 8) * 8;
+    std::vector<int32_t> tslot(static_cast<size_t>(n_blocks * kScsThreads), -1);
+    for (size_t t = 0; t < tail.size(); ++t) tslot[pos[t]] = tail[t].second;
+    std::vector<int64_t> tptr(static_cast<size_t>(n_blocks * per_block), 0);
+    int64_t store = 0;
+    for (int64_t w = 0; w < n_waves; ++w) {
+        tptr[w] = store;
+        store += wave_len[w] * kWave;
+    }
+    smax_hip_scs_tails t;
+    t.n_elements = store;
+    t.n_slots = static_cast<int64_t>(tail.size());
+    t.n_waves = n_waves;
+    int rc = 0;
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&t.head), sizeof(int32_t) * nc, "scs tails");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&t.slot), sizeof(int32_t) * tslot.size(),
+                    "scs tails");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&t.ptr), sizeof(int64_t) * tptr.size(),
+                    "scs tails");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&t.col), sizeof(int32_t) * store, "scs tails");
+    rc |= dev_alloc(ctx, &t.val, static_cast<size_t>(in.val_bytes) * store, "scs tails");
+    if (rc) {
+        free_tails(t);
+        return SMAX_HIP_ERROR_OUT_OF_MEMORY;
+    }
+    // elements past a lane's length are never read; zero them all the same
+    rc = smax_hip_zero(ctx, t.col, sizeof(int32_t) * store);
+    if (!rc) rc = smax_hip_zero(ctx, t.val, static_cast<size_t>(in.val_bytes) * store);
+    if (rc) {
+        free_tails(t);
+        return rc;
+    }
+    const hipError_t e1 = hipMemcpyAsync(t.head, head.data(), sizeof(int32_t) * nc,
+                                         hipMemcpyHostToDevice, ctx->stream);
+    const hipError_t e2 = hipMemcpyAsync(t.slot, tslot.data(), sizeof(int32_t) * tslot.size(),
+                                         hipMemcpyHostToDevice, ctx->stream);
+    const hipError_t e3 = hipMemcpyAsync(t.ptr, tptr.data(), sizeof(int64_t) * tptr.size(),
+                                         hipMemcpyHostToDevice, ctx->stream);
+    const hipError_t e4 = hipStreamSynchronize(ctx->stream);
+    for (hipError_t e : {e1, e2, e3, e4})
+        if (e != hipSuccess) {
+            free_tails(t);
+            return hip_fail(ctx, e, "scs tails");
+        }
+    t.n_blocks = n_blocks;
+    A->tails = t;  // freed with the handle from here on
+    rc = tail_fill(ctx, A, true);
+    if (rc) return rc;
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    return SMAX_HIP_SUCCESS;
+}
+
+int refresh_tail_values(smax_hip_ctx ctx, smax_hip_scs A) {
+    return A->tails.n_blocks > 0 ? tail_fill(ctx, A, false) : SMAX_HIP_SUCCESS;
+}
+
 static int check_scs_args(smax_hip_ctx ctx, const char *fn, int value_type,
                           int64_t C, int64_t n_chunks, const void *chunk_ptr,
                           const void *chunk_len, const void *col,
@@ -763,8 +964,18 @@ int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                                                     in.C <= kScsThreads
                                                         ? A->sched.order
                                                         : A->sched.order_lf,
-                                                    A->sched.lds_cap_bytes));
+                                                    A->sched.lds_cap_bytes, &A->tails));
     return SMAX_HIP_ERROR_UNSUPPORTED;
+}
+
+int smax_hip_scs_get_tails(smax_hip_scs A, int64_t *n_slots, int64_t *n_waves,
+                           int64_t *n_entries) {
+    if (!A || !n_slots || !n_waves || !n_entries)
+        return fail(nullptr, SMAX_HIP_ERROR_INVALID_VALUE, "smax_hip_scs_get_tails: null");
+    *n_slots = A->tails.n_slots;
+    *n_waves = A->tails.n_waves;
+    *n_entries = A->tails.n_elements;
+    return SMAX_HIP_SUCCESS;
 }
 
 int smax_hip_scs_spmm(smax_hip_ctx ctx, int value_type, int64_t C,

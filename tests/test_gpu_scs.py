@@ -249,6 +249,51 @@ def test_scs_spmv_plan_on_caller_arrays(ctx, C_, sigma):
     s.destroy()
 
 
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("sigma", [1, 64])
+@pytest.mark.parametrize("x0", [0.5, np.nan])
+def test_scs_apply_tail_waves(ctx, vt, sigma, x0):
+    """Heavy-tailed rows in unsorted chunks pad the layout ~17x: the long
+    slots of skewed chunks then run in tail waves of their own.  The result
+    is still the full padded chain of every slot, bit for bit, in both output
+    orders (a NaN x[0] reaches every padded slot through its padding term,
+    tail slots included), and again after a value refresh, which must reach
+    the tail waves' copies.  A sigma-sorted layout of the same matrix gets
+    no tail waves."""
+    A = matgen.lomax_band(60000, half=7500)
+    val, x = values_for(vt, A)
+    x = x.copy()
+    x[0] = x0
+    s = gpu_scs(ctx, A, 64, sigma, val)
+    slots, waves, entries = s.tails()
+    assert slots > 0 and 0 < waves <= slots and entries >= 64 * waves
+    dx = ctx.upload(x)
+    dy = ctx.zeros(s.info.n_rows_padded, x.dtype)
+    dy2 = ctx.zeros(A.n_rows, x.dtype)
+    for rnd in range(2):
+        if rnd:
+            new = matgen.uniform_f64(78, A.nnz)
+            val = {capi.F64: new, capi.F32: new.astype(np.float32),
+                   capi.BF16: matgen.to_bf16(new)}[vt]
+            dv = ctx.upload(val)
+            s.refresh_values(dv)
+            dv.free()
+        want_s = oracle.crs_to_scs(A.row_ptr, A.col, val, 64, sigma)
+        want = oracle.scs_spmv(64, want_s["n_chunks"], want_s["chunk_ptr"],
+                               want_s["chunk_lengths"], want_s["col"], want_s["val"], x)
+        s.apply(vt, dx, dy)
+        assert_same_values(ctx.download(dy), want, f"apply with tail waves, round {rnd}")
+        s.apply(vt, dx, dy2, unpermuted=True)
+        assert_same_values(ctx.download(dy2), want[want_s["perm"]],
+                           f"apply with tail waves, original order, round {rnd}")
+    for d in (dx, dy, dy2):
+        d.free()
+    s.destroy()
+    s2 = gpu_scs(ctx, A, 64, 4096, val)
+    assert s2.tails() == (0, 0, 0)
+    s2.destroy()
+
+
 def test_scs_apply_rejects_mismatched_value_type(ctx):
     A = matgen.poisson2d(16)
     s = gpu_scs(ctx, A, 64, 1)
