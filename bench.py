@@ -875,7 +875,9 @@ def main() -> int:
                              "GFLOPs": round(2 * A.nnz / ms / 1e6, 1),
                              "GBs_format": round(fmt / ms / 1e6, 1),
                              "fill": round(s.info.n_elements / A.nnz, 4),
-                             "conversion_s": round(conv_s, 3)}
+                             "conversion_s": round(conv_s, 3),
+                             "tail_waves": dict(zip(("slots", "waves", "entries"),
+                                                    s.tails()))}
                 log(f"SELL {key}: {ms:.4f} ms {gbs:.1f} GB/s")
                 sell[key]["sigma_listed"] = sigma in (1, 64, 4096, A.n_rows)
                 if sell[key]["sigma_listed"] and (best is None or ms < best[1]):

@@ -318,14 +318,14 @@ int smax_hip_scs_apply(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                        const void *x, void *y, int unpermuted);
 
 /*
- * How smax_hip_scs_apply runs A (C = 64): the slots moved out of chunks
- * that mix rows of very different lengths into tail waves of their own
- * (n_slots; 0 when the layout has none), the number of those waves, and the
- * entries they store (copies of those slots' elements, with padding).  A
- * layout gets tail waves when its padding exceeds 3x its stored entries; in
- * such a chunk the slots longer than its 8th-longest slot move when that
- * slot is under half the chunk's length.  Results are unchanged bit for bit.
- * No reference counterpart.
+ * How smax_hip_scs_apply runs A: the slots moved out of chunks that mix rows
+ * of very different lengths into tail waves of their own (n_slots; 0 when
+ * the layout has none), the number of those waves, and the entries they
+ * store (copies of those slots' elements, with padding).  A layout with
+ * 16 <= C <= 256 (C a multiple of 8) gets tail waves when its padding exceeds
+ * 5x its stored entries; in such a chunk the slots longer than its (C/8)-th
+ * longest slot move when that slot is under half the chunk's length.
+ * Results are unchanged bit for bit.  No reference counterpart.
  */
 int smax_hip_scs_get_tails(smax_hip_scs A, int64_t *n_slots, int64_t *n_waves,
                            int64_t *n_entries);

@@ -17,16 +17,16 @@ struct smax_hip_scs_sched {
     int64_t lds_cap_bytes = 0;
 };
 
-// Tail waves of a C = 64 layout whose chunks mix rows of very different
-// lengths (built with the conversion when padding exceeds 3x the stored
-// entries).  In a chunk whose 8th-longest slot is under half the chunk's
-// length, the slots longer than that 8th-longest one are taken out of the
-// chunk's wave and repacked, longest first, into waves of their own: a SELL-64
-// of their own whose element j of lane l of wave w sits at ptr[w] + j*64 + l,
-// a wave taking slots while it has fewer than 64 and their lengths sum to at
-// most kTailWaveWork.  The chunk's wave then runs only as long as its 8th-
-// longest slot.  Each slot's chain is still one lane's sequential chain over
-// the same elements in the same order.
+// Tail waves of a layout whose chunks mix rows of very different lengths
+// (built with the conversion for 16 <= C <= 256, C a multiple of 8, when
+// padding exceeds 5x the stored entries).  In a chunk whose (C/8)-th longest
+// slot is under half the chunk's length, the slots longer than that one are
+// taken out of the chunk and repacked, longest first, into waves of their
+// own: a SELL-64 whose element j of lane l of wave w sits at
+// ptr[w] + j*64 + l, a wave taking slots while it has fewer than 64 and their
+// lengths sum to at most kTailWaveWork.  The chunk then runs only as long as
+// its (C/8)-th longest slot.  Each slot's chain is still one lane's
+// sequential chain over the same elements in the same order.
 struct smax_hip_scs_tails {
     int32_t *head = nullptr;  // per chunk: longer slots run in tail waves
     int32_t *slot = nullptr;  // per tail-wave lane (n_blocks * 256): slot, -1 none

@@ -160,7 +160,8 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
 
 // Any C: one lane per slot, 256 / C chunks per workgroup (C <= 256) or one
 // chunk over several workgroups; chunks in the optional execution `order`;
-// LEN as in scs_spmv_w64.
+// LEN and the tail waves (C <= 256 only; their arrays are a SELL-64 whatever
+// C is) as in scs_spmv_w64.
 template <int VT, bool UNPERM, bool LEN, bool REMAP>
 __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
     int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
@@ -170,27 +171,56 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
     const typename vt<VT>::val_t *__restrict__ val,
     const int32_t *__restrict__ slot_row,
     const typename vt<VT>::vec_t *__restrict__ x,
-    typename vt<VT>::vec_t *__restrict__ y) {
+    typename vt<VT>::vec_t *__restrict__ y, const int32_t *__restrict__ t_head,
+    const int32_t *__restrict__ t_slot, const int64_t *__restrict__ t_ptr,
+    const int32_t *__restrict__ t_col, const typename vt<VT>::val_t *__restrict__ t_val,
+    int64_t t_blocks) {
     using T = vt<VT>;
-    int64_t pos, i;
-    if (cpb > 0) {
-        const int64_t blk = REMAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-        pos = blk * cpb + threadIdx.x / C;
-        i = threadIdx.x % C;
-        if (threadIdx.x >= cpb * C) return;
+    const bool tails = LEN && t_slot != nullptr;
+    int64_t slot, base, stride;
+    int clen, len;
+    const int32_t *cp;
+    const typename T::val_t *vp;
+    if (tails && static_cast<int64_t>(blockIdx.x) < t_blocks) {
+        const int lane = threadIdx.x & (kWave - 1);
+        const int64_t w = static_cast<int64_t>(blockIdx.x) * (kScsThreads / kWave) +
+                          threadIdx.x / kWave;
+        const int32_t ts = t_slot[w * kWave + lane];
+        if (ts < 0) return;
+        slot = ts;
+        base = t_ptr[w] + lane;
+        stride = kWave;
+        clen = chunk_len[ts / C];
+        len = slot_len[ts];
+        cp = t_col;
+        vp = t_val;
     } else {
-        pos = blockIdx.x / bpc;
-        i = (blockIdx.x % bpc) * kScsThreads + threadIdx.x;
-        if (i >= C) return;
+        int64_t pos, i;
+        if (cpb > 0) {
+            const int64_t hb = tails ? blockIdx.x - t_blocks : blockIdx.x;
+            const int64_t hn = tails ? gridDim.x - t_blocks : gridDim.x;
+            const int64_t blk = REMAP ? xcd_block(hb, hn) : hb;
+            pos = blk * cpb + threadIdx.x / C;
+            i = threadIdx.x % C;
+            if (threadIdx.x >= cpb * C) return;
+        } else {
+            pos = blockIdx.x / bpc;
+            i = (blockIdx.x % bpc) * kScsThreads + threadIdx.x;
+            if (i >= C) return;
+        }
+        if (pos >= n_chunks) return;
+        const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
+        if (chunk < 0) return;
+        slot = chunk * C + i;
+        base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
+        stride = C;
+        clen = chunk_len[chunk];
+        len = LEN ? slot_len[slot] : clen;
+        if (tails && len > t_head[chunk]) return;
+        cp = col;
+        vp = val;
     }
-    if (pos >= n_chunks) return;
-    const int64_t chunk = order ? static_cast<int64_t>(order[pos]) : pos;
-    if (chunk < 0) return;
-    const int64_t slot = chunk * C + i;
-    const int64_t base = static_cast<int64_t>(chunk_ptr[chunk]) + i;
-    const int clen = chunk_len[chunk];
-    const int len = LEN ? slot_len[slot] : clen;
-    auto acc = slot_chain_pipe<VT, kScsUnroll>(col, val, x, base, C, len);
+    auto acc = slot_chain_pipe<VT, kScsUnroll>(cp, vp, x, base, stride, len);
     if (LEN && len < clen) acc = T::fma_(T::widen(typename T::val_t(0)), x[0], acc);
     if (UNPERM) {
         const int32_t r = slot_row[slot];
@@ -514,14 +544,20 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
     auto yv = static_cast<typename T::vec_t *>(y);
     const int64_t cpb = C <= kScsThreads ? kScsThreads / C : 0;
     const int64_t bpc = C <= kScsThreads ? 1 : ceil_div(C, kScsThreads);
-    const int64_t blocks = cpb > 0 ? ceil_div(n_chunks, cpb) : n_chunks * bpc;
+    const smax_hip_scs_tails *t =
+        slot_len && cpb > 0 && tails && tails->n_blocks > 0 ? tails : nullptr;
+    const int64_t tb = t ? t->n_blocks : 0;
+    const int64_t blocks = (cpb > 0 ? ceil_div(n_chunks, cpb) : n_chunks * bpc) + tb;
     SMAX_HIP_REQUIRE(ctx, blocks <= INT32_MAX, "scs spmv: grid too large");
     const dim3 grid(static_cast<unsigned>(blocks));
     const bool rm = remap && cpb > 0;
-#define SMAX_SCS_GENERIC(U, L, R)                                                    \
-    hipLaunchKernelGGL((scs_spmv_generic<VT, U, L, R>), grid, dim3(kScsThreads), 0, \
-                       ctx->stream, C, n_chunks, n_rows, cpb, bpc, chunk_ptr,       \
-                       chunk_len, slot_len, order, col, v, slot_row, xv, yv)
+    auto tv = static_cast<const typename T::val_t *>(t ? t->val : nullptr);
+#define SMAX_SCS_GENERIC(U, L, R)                                                        \
+    hipLaunchKernelGGL((scs_spmv_generic<VT, U, L, R>), grid, dim3(kScsThreads), 0,     \
+                       ctx->stream, C, n_chunks, n_rows, cpb, bpc, chunk_ptr,           \
+                       chunk_len, slot_len, order, col, v, slot_row, xv, yv,            \
+                       t ? t->head : nullptr, t ? t->slot : nullptr, t ? t->ptr : nullptr, \
+                       t ? t->col : nullptr, tv, tb)
     if (unperm && slot_len) {
         if (rm) SMAX_SCS_GENERIC(true, true, true); else SMAX_SCS_GENERIC(true, true, false);
     } else if (unperm) {
@@ -668,20 +704,22 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
 }
 
 // ------------------------------------------------------------ tail waves
-// Layouts whose padding exceeds 3x their stored entries get tail waves (see
+// Layouts whose padding exceeds 5x their stored entries get tail waves (see
 // smax_hip_scs_tails).  Measured on the 4M-row band matrix (FP64): SELL-64-1
-// 2.32 -> 1.10 ms, SELL-64-64 2.09 -> 1.04 ms.  With less padding the long
-// slots are few, their waves carry the kernel's longest chains together and
-// finish last: SELL-64-4096 (2.5x) took 0.76 ms against 0.70 without.
-constexpr int kTailRank = 8;              // a chunk's head runs to its 8th-longest slot
+// 2.32 -> 1.14 ms, SELL-64-64 2.09 -> 1.04 ms, SELL-32-1 2.14 -> 1.17 ms,
+// SELL-32-64 1.91 -> 1.05 ms, SELL-128-1 2.20 -> 1.13 ms.  Sorted windows
+// leave only the matrix's longest rows skewed: their waves then carry the
+// kernel's longest chains together and finish last (SELL-64-4096, fill 2.5:
+// 0.76 ms against 0.70; SELL-128-4096, fill 4.0: 0.82 against 0.75).
+// a chunk's head runs to its (C/8)-th longest slot (the 8th for C = 64)
 constexpr int64_t kTailWaveWork = 32768;  // elements per tail wave at most
 
 // Copies the elements of every tail slot into its wave's arrays: element j of
-// the slot (chunk c, lane i) at chunk_ptr[c] + j*64 + i goes to
+// the slot (chunk c, lane i) at chunk_ptr[c] + j*C + i goes to
 // t_ptr[w] + j*64 + l for its tail-wave lane l of wave w.  One thread per
 // tail-wave lane; t_col null: values only.
 template <typename E>
-__global__ void k_tail_fill(int64_t n_lanes, const int32_t *__restrict__ t_slot,
+__global__ void k_tail_fill(int64_t n_lanes, int64_t C, const int32_t *__restrict__ t_slot,
                             const int64_t *__restrict__ t_ptr,
                             const int32_t *__restrict__ slot_len,
                             const int32_t *__restrict__ chunk_ptr,
@@ -691,13 +729,13 @@ __global__ void k_tail_fill(int64_t n_lanes, const int32_t *__restrict__ t_slot,
     if (p >= n_lanes) return;
     const int32_t ts = t_slot[p];
     if (ts < 0) return;
-    const int64_t src = static_cast<int64_t>(chunk_ptr[ts / kWave]) + ts % kWave;
+    const int64_t src = static_cast<int64_t>(chunk_ptr[ts / C]) + ts % C;
     const int64_t dst = t_ptr[p / kWave] + p % kWave;
     const int len = slot_len[ts];
     for (int j = 0; j < len; ++j) {
-        const int64_t o = static_cast<int64_t>(j) * kWave;
-        if (t_col) t_col[dst + o] = col[src + o];
-        t_val[dst + o] = val[src + o];
+        const int64_t s = static_cast<int64_t>(j) * C, d = static_cast<int64_t>(j) * kWave;
+        if (t_col) t_col[dst + d] = col[src + s];
+        t_val[dst + d] = val[src + s];
     }
 }
 
@@ -708,7 +746,8 @@ static int tail_fill(smax_hip_ctx ctx, smax_hip_scs A, bool with_col) {
     int32_t *tc = with_col ? t.col : nullptr;
 #define SMAX_TAIL_FILL(E)                                                                 \
     hipLaunchKernelGGL((k_tail_fill<E>), grid_for(lanes, kScsThreads), dim3(kScsThreads), \
-                       0, ctx->stream, lanes, t.slot, t.ptr, in.slot_len, in.chunk_ptr,   \
+                       0, ctx->stream, lanes, in.C, t.slot, t.ptr, in.slot_len,           \
+                       in.chunk_ptr,                                                      \
                        in.col, static_cast<const E *>(in.val), tc, static_cast<E *>(t.val))
     switch (in.val_bytes) {
     case 8: SMAX_TAIL_FILL(uint64_t); break;
@@ -731,9 +770,12 @@ void free_tails(smax_hip_scs_tails &t) {
 
 int build_tails(smax_hip_ctx ctx, smax_hip_scs A) {
     const smax_hip_scs_info &in = A->info;
-    if (in.C != kWave || in.nnz <= 0 || in.n_elements <= 4 * in.nnz) return SMAX_HIP_SUCCESS;
+    const int64_t C = in.C;
+    if (C < 16 || C > kScsThreads || C % 8 != 0 || in.nnz <= 0 || in.n_elements <= 6 * in.nnz)
+        return SMAX_HIP_SUCCESS;
+    const int64_t rank = C / 8;
     const int64_t nc = in.n_chunks;
-    std::vector<int32_t> sl(static_cast<size_t>(nc * kWave)), cl(static_cast<size_t>(nc));
+    std::vector<int32_t> sl(static_cast<size_t>(nc * C)), cl(static_cast<size_t>(nc));
     SMAX_HIP_CALL(ctx, hipMemcpyAsync(sl.data(), in.slot_len, sizeof(int32_t) * sl.size(),
                                       hipMemcpyDeviceToHost, ctx->stream));
     SMAX_HIP_CALL(ctx, hipMemcpyAsync(cl.data(), in.chunk_lengths, sizeof(int32_t) * nc,
@@ -742,15 +784,16 @@ int build_tails(smax_hip_ctx ctx, smax_hip_scs A) {
     // per chunk: the head length, and the slots beyond it
     std::vector<int32_t> head(static_cast<size_t>(nc));
     std::vector<std::pair<int32_t, int32_t>> tail;  // (length, slot)
-    int32_t lens[kWave];
+    std::vector<int32_t> lens(static_cast<size_t>(C));
     for (int64_t c = 0; c < nc; ++c) {
-        const int32_t *l = sl.data() + c * kWave;
-        std::copy(l, l + kWave, lens);
-        std::nth_element(lens, lens + kTailRank - 1, lens + kWave, std::greater<int32_t>());
-        const int32_t h = lens[kTailRank - 1];
+        const int32_t *l = sl.data() + c * C;
+        std::copy(l, l + C, lens.begin());
+        std::nth_element(lens.begin(), lens.begin() + (rank - 1), lens.end(),
+                         std::greater<int32_t>());
+        const int32_t h = lens[rank - 1];
         head[c] = 2 * static_cast<int64_t>(h) < cl[c] ? h : cl[c];
-        for (int i = 0; i < kWave; ++i)
-            if (l[i] > head[c]) tail.emplace_back(l[i], static_cast<int32_t>(c * kWave + i));
+        for (int64_t i = 0; i < C; ++i)
+            if (l[i] > head[c]) tail.emplace_back(l[i], static_cast<int32_t>(c * C + i));
     }
     if (tail.empty()) return SMAX_HIP_SUCCESS;
     // longest first (ties in slot order), packed greedily into waves

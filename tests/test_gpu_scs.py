@@ -250,21 +250,21 @@ def test_scs_spmv_plan_on_caller_arrays(ctx, C_, sigma):
 
 
 @pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
-@pytest.mark.parametrize("sigma", [1, 64])
+@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 64), (32, 64), (128, 1)])
 @pytest.mark.parametrize("x0", [0.5, np.nan])
-def test_scs_apply_tail_waves(ctx, vt, sigma, x0):
-    """Heavy-tailed rows in unsorted chunks pad the layout ~17x: the long
-    slots of skewed chunks then run in tail waves of their own.  The result
-    is still the full padded chain of every slot, bit for bit, in both output
-    orders (a NaN x[0] reaches every padded slot through its padding term,
-    tail slots included), and again after a value refresh, which must reach
-    the tail waves' copies.  A sigma-sorted layout of the same matrix gets
-    no tail waves."""
+def test_scs_apply_tail_waves(ctx, vt, C_, sigma, x0):
+    """Heavy-tailed rows in unsorted chunks pad the layout 9-17x: the long
+    slots of skewed chunks then run in tail waves of their own (a SELL-64
+    whatever C is).  The result is still the full padded chain of every slot,
+    bit for bit, in both output orders (a NaN x[0] reaches every padded slot
+    through its padding term, tail slots included), and again after a value
+    refresh, which must reach the tail waves' copies.  The fully sorted
+    layout (sigma = N) of the same matrix gets no tail waves."""
     A = matgen.lomax_band(60000, half=7500)
     val, x = values_for(vt, A)
     x = x.copy()
     x[0] = x0
-    s = gpu_scs(ctx, A, 64, sigma, val)
+    s = gpu_scs(ctx, A, C_, sigma, val)
     slots, waves, entries = s.tails()
     assert slots > 0 and 0 < waves <= slots and entries >= 64 * waves
     dx = ctx.upload(x)
@@ -278,8 +278,8 @@ def test_scs_apply_tail_waves(ctx, vt, sigma, x0):
             dv = ctx.upload(val)
             s.refresh_values(dv)
             dv.free()
-        want_s = oracle.crs_to_scs(A.row_ptr, A.col, val, 64, sigma)
-        want = oracle.scs_spmv(64, want_s["n_chunks"], want_s["chunk_ptr"],
+        want_s = oracle.crs_to_scs(A.row_ptr, A.col, val, C_, sigma)
+        want = oracle.scs_spmv(C_, want_s["n_chunks"], want_s["chunk_ptr"],
                                want_s["chunk_lengths"], want_s["col"], want_s["val"], x)
         s.apply(vt, dx, dy)
         assert_same_values(ctx.download(dy), want, f"apply with tail waves, round {rnd}")
@@ -289,7 +289,7 @@ def test_scs_apply_tail_waves(ctx, vt, sigma, x0):
     for d in (dx, dy, dy2):
         d.free()
     s.destroy()
-    s2 = gpu_scs(ctx, A, 64, 4096, val)
+    s2 = gpu_scs(ctx, A, C_, A.n_rows, val)
     assert s2.tails() == (0, 0, 0)
     s2.destroy()
 
