@@ -710,9 +710,12 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
 // SELL-32-64 1.91 -> 1.05 ms, SELL-128-1 2.20 -> 1.13 ms.  Sorted windows
 // leave only the matrix's longest rows skewed: their waves then carry the
 // kernel's longest chains together and finish last (SELL-64-4096, fill 2.5:
-// 0.76 ms against 0.70; SELL-128-4096, fill 4.0: 0.82 against 0.75).
-// a chunk's head runs to its (C/8)-th longest slot (the 8th for C = 64)
-constexpr int64_t kTailWaveWork = 32768;  // elements per tail wave at most
+// 0.76 ms against 0.70; SELL-128-4096, fill 4.0: 0.82 against 0.75).  A
+// skewed chunk keeps its slots up to its (C/8)-th longest one (the 8th for
+// C = 64: 1.10 ms for SELL-64-1, against 1.23 ms up to the 4th);
+// a tail wave takes at most kTailWaveWork elements (32768: 1.10 ms, 16384:
+// 1.11 ms, unbounded 1.39 ms).
+constexpr int64_t kTailWaveWork = 32768;
 
 // Copies the elements of every tail slot into its wave's arrays: element j of
 // the slot (chunk c, lane i) at chunk_ptr[c] + j*C + i goes to
