@@ -300,6 +300,28 @@ int smax_hip_scs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
                           const int32_t *col, const void *val,
                           int64_t val_bytes, int64_t C, int64_t sigma,
                           smax_hip_scs *out);
+
+/*
+ * A handle over SELL-C-sigma arrays the caller already holds on the device
+ * (the 13 values SpMVKernel::register_A takes for a SELL matrix,
+ * src/kernels/spmv.hpp:37-50, with val_bytes in {2, 4, 8}): the arrays are
+ * copied, so the caller's may change or go away afterwards.  It derives what
+ * smax_hip_scs_apply needs from them -- each slot's row from perm, each
+ * slot's length (its elements up to the last one that is not a +0.0 value at
+ * column 0: such trailing entries add exactly the padding term 0 * x[0], so
+ * the product is unchanged bit for bit), the execution schedule and the tail
+ * waves -- and refuses arrays whose chunk_ptr does not match
+ * C * chunk_lengths, whose column indices leave [0, n_cols) or whose perm is
+ * not a permutation into [0, n_rows_padded).  Only smax_hip_scs_apply,
+ * _get_info, _get_tails and _destroy take such a handle (no CRS positions:
+ * no value refresh, SDDMM or solvers; no SpMM).  No reference counterpart:
+ * it is how the SMAX::Interface HIP platform runs registered SELL matrices.
+ */
+int smax_hip_scs_from_sell(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows,
+                           int64_t n_rows_padded, int64_t n_cols, int64_t n_chunks,
+                           int64_t n_elements, int64_t nnz, const int32_t *chunk_ptr,
+                           const int32_t *chunk_lengths, const int32_t *col, const void *val,
+                           int64_t val_bytes, const int32_t *perm, smax_hip_scs *out);
 int smax_hip_scs_get_info(smax_hip_scs scs, smax_hip_scs_info *info);
 int smax_hip_scs_destroy(smax_hip_ctx ctx, smax_hip_scs scs);
 

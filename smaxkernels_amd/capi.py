@@ -110,6 +110,8 @@ _SIGS = {
     "smax_hip_scs_destroy": ([vp, vp], C.c_int),
     "smax_hip_scs_refresh_values": ([vp, vp, vp], C.c_int),
     "smax_hip_scs_get_tails": ([vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)], C.c_int),
+    "smax_hip_scs_from_sell": ([vp, i64, i64, i64, i64, i64, i64, i64, i64, vp, vp, vp, vp, i64,
+                                vp, C.POINTER(vp)], C.c_int),
     "smax_hip_scs_apply": ([vp, C.c_int, vp, vp, vp, C.c_int], C.c_int),
     "smax_hip_bcrs_spmv": ([vp, C.c_int, i64, i64, i64, i64, i64, vp, vp, vp,
                             C.c_int, vp, vp], C.c_int),
@@ -329,6 +331,16 @@ class Context:
             self.handle, vtype, plan, _ptr(chunk_ptr), _ptr(chunk_len), _ptr(col),
             _ptr(val), _ptr(x), _ptr(y)))
 
+    def scs_from_sell(self, C_, sigma, n_rows, n_rows_padded, n_cols, n_chunks, n_elements,
+                      nnz, chunk_ptr, chunk_lengths, col, val, val_bytes, perm) -> "Scs":
+        """A SpMV-only handle over device SELL arrays (copied)."""
+        h = vp()
+        self._check(self.lib.smax_hip_scs_from_sell(
+            self.handle, C_, sigma, n_rows, n_rows_padded, n_cols, n_chunks, n_elements, nnz,
+            _ptr(chunk_ptr), _ptr(chunk_lengths), _ptr(col), _ptr(val), val_bytes, _ptr(perm),
+            C.byref(h)))
+        return Scs(self, h.value)
+
     def scs_from_crs(self, n_rows, n_cols, nnz, row_ptr, col, val, val_bytes,
                      C_, sigma) -> "Scs":
         h = vp()
@@ -416,7 +428,7 @@ class Scs:
             "val": d(i.val, i.n_elements, dtype),
             "perm": d(i.perm, i.n_rows, np.int32),
             "slot_row": d(i.slot_row, i.n_rows_padded, np.int32),
-            "src_index": d(i.src_index, i.n_elements, np.int32),
+            "src_index": d(i.src_index, i.n_elements, np.int32) if i.src_index else None,
             "slot_len": d(i.slot_len, i.n_rows_padded, np.int32),
         }
 

@@ -400,6 +400,9 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                     "smax_hip_scs_sddmm: value type %d", value_type);
     SMAX_HIP_REQUIRE(ctx, A && k > 0 && G && X && dvals,
                      "smax_hip_scs_sddmm: bad arguments");
+    SMAX_HIP_REQUIRE(ctx, !A->from_sell,
+                     "smax_hip_scs_sddmm: the matrix was built from SELL arrays and has no CRS "
+                     "positions; build it with smax_hip_scs_from_crs");
     const smax_hip_scs_info &in = A->info;
     if (in.n_elements == 0) return SMAX_HIP_SUCCESS;
     // the row kernel's 16-byte loads need 16-byte aligned G and X rows

@@ -987,6 +987,9 @@ static int check_solver_args(smax_hip_ctx ctx, const char *fn, smax_hip_scs A,
                              const smax_hip_cg_params *prm,
                              smax_hip_cg_report *rep) {
     SMAX_HIP_REQUIRE(ctx, A && b && x && prm && rep, "%s: null argument", fn);
+    SMAX_HIP_REQUIRE(ctx, !A->from_sell,
+                     "%s: the matrix was built from SELL arrays; build it with "
+                     "smax_hip_scs_from_crs", fn);
     SMAX_HIP_REQUIRE(ctx, A->info.val_bytes == 8,
                      "%s: the matrix must hold FP64 values", fn);
     SMAX_HIP_REQUIRE(ctx, A->info.n_rows == A->info.n_cols,

@@ -72,7 +72,11 @@ struct smax_hip_scs_s {
     // Built on first use by the SDDMM: row of every stored element (-1 for
     // padding), so an element's row needs no search.
     int32_t *elem_row = nullptr;
-    smax_hip_scs_tails tails;     // built with the conversion (C = 64 only)
+    smax_hip_scs_tails tails;     // built with the conversion
+    // Built by smax_hip_scs_from_sell from caller SELL arrays: no src_index
+    // (no CRS positions), slot lengths derived from the arrays.  Only the SpMV
+    // runs on such a handle.
+    bool from_sell = false;
 };
 
 struct smax_hip_scs_plan_s {

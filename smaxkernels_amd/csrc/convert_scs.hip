@@ -440,6 +440,79 @@ __global__ void k_refresh(int64_t n_elem, const int32_t *__restrict__ src,
     val[e] = s >= 0 ? crs_val[s] : V(0);
 }
 
+// ------------------------------------------------------- caller SELL arrays
+// Structure checks of SELL arrays handed in by a caller: every chunk's range
+// is C * chunk_lengths[c] entries from chunk_ptr[c] (chunk_ptr[0] = 0,
+// chunk_ptr[n_chunks] = n_elements) and every column lies in [0, n_cols).
+// Any failure sets *bad (every writer stores the same 1).
+__global__ void k_check_sell_chunks(int64_t n_chunks, int64_t C, int64_t n_elements,
+                                    const int32_t *__restrict__ chunk_ptr,
+                                    const int32_t *__restrict__ chunk_len,
+                                    int32_t *__restrict__ bad) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c > n_chunks) return;
+    if (c == 0 && chunk_ptr[0] != 0) *bad = 1;
+    if (c == n_chunks) {
+        if (chunk_ptr[c] != n_elements) *bad = 1;
+        return;
+    }
+    const int64_t len = chunk_len[c];
+    if (len < 0 || static_cast<int64_t>(chunk_ptr[c + 1]) - chunk_ptr[c] != C * len) *bad = 1;
+}
+
+__global__ void k_check_sell_cols(int64_t n_elements, int64_t n_cols,
+                                  const int32_t *__restrict__ col, int32_t *__restrict__ bad) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < n_elements && (col[e] < 0 || col[e] >= n_cols)) *bad = 1;
+}
+
+// slot_row[perm[r]] = r (slot_row prefilled with n_rows, which marks padding);
+// perm entries outside [0, n_slots) set *bad and are skipped.
+__global__ void k_slot_row_from_perm(int64_t n_rows, int64_t n_slots,
+                                     const int32_t *__restrict__ perm,
+                                     int32_t *__restrict__ slot_row, int32_t *__restrict__ bad) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    const int32_t s = perm[r];
+    if (s < 0 || s >= n_slots) {
+        *bad = 1;
+        return;
+    }
+    slot_row[s] = static_cast<int32_t>(r);
+}
+
+// After the scatter: a perm that maps two rows to one slot leaves one of them
+// unmatched.
+__global__ void k_check_perm(int64_t n_rows, const int32_t *__restrict__ perm,
+                             const int32_t *__restrict__ slot_row, int32_t *__restrict__ bad) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < n_rows && slot_row[perm[r]] != r) *bad = 1;
+}
+
+// Stored length of each slot of caller SELL arrays: the slot's elements up to
+// its last one that is not +0.0 (all value bits zero) at column 0.  Trailing
+// +0.0 entries at column 0 are exactly the reference's padding elements: each
+// adds fma(+0, x[0], acc), the padding term, which a lane that stops early
+// applies once (it is idempotent), so the product is unchanged bit for bit.
+// A trailing -0.0 stays: fma(-0, x, -0) = -0 where fma(+0, x, -0) = +0.
+template <typename E>
+__global__ void k_sell_slot_len(int64_t n_slots, int64_t C,
+                                const int32_t *__restrict__ chunk_ptr,
+                                const int32_t *__restrict__ chunk_len,
+                                const int32_t *__restrict__ col, const E *__restrict__ val,
+                                int32_t *__restrict__ slot_len) {
+    const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    const int64_t c = s / C, base = static_cast<int64_t>(chunk_ptr[c]) + s % C;
+    int j = chunk_len[c];
+    while (j > 0) {
+        const int64_t e = base + static_cast<int64_t>(j - 1) * C;
+        if (val[e] != E(0) || col[e] != 0) break;
+        --j;
+    }
+    slot_len[s] = j;
+}
+
 // ------------------------------------------------------------ host side
 // Temporary device buffers, released (after the stream drains) on scope exit.
 struct Scratch {
@@ -691,11 +764,148 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
     return SMAX_HIP_SUCCESS;
 }
 
+int build_scs_from_sell(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows,
+                        int64_t n_rows_padded, int64_t n_cols, int64_t n_chunks,
+                        int64_t n_elements, int64_t nnz, const int32_t *chunk_ptr,
+                        const int32_t *chunk_lengths, const int32_t *col, const void *val,
+                        int64_t val_bytes, const int32_t *perm, smax_hip_scs *out) {
+    SMAX_HIP_REQUIRE(ctx, out, "smax_hip_scs_from_sell: null output");
+    *out = nullptr;
+    SMAX_HIP_REQUIRE(ctx, C > 0 && sigma > 0 && n_chunks > 0 && n_rows_padded == n_chunks * C &&
+                              n_rows > 0 && n_rows <= n_rows_padded && n_cols > 0 &&
+                              n_elements >= 0 && nnz >= 0 && nnz <= n_elements,
+                     "smax_hip_scs_from_sell: inconsistent sizes");
+    SMAX_HIP_REQUIRE(ctx, val_bytes == 2 || val_bytes == 4 || val_bytes == 8,
+                     "smax_hip_scs_from_sell: val_bytes must be 2, 4 or 8");
+    SMAX_HIP_REQUIRE(ctx, chunk_ptr && chunk_lengths && perm && (n_elements == 0 || (col && val)),
+                     "smax_hip_scs_from_sell: null array");
+    if (n_rows_padded > INT32_MAX || n_elements > INT32_MAX)
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW, "smax_hip_scs_from_sell: sizes exceed int32");
+    smax_hip_scs_info info{};
+    info.C = C;
+    info.sigma = sigma;
+    info.n_rows = n_rows;
+    info.n_rows_padded = n_rows_padded;
+    info.n_cols = n_cols;
+    info.n_chunks = n_chunks;
+    info.n_elements = n_elements;
+    info.nnz = nnz;
+    info.val_bytes = val_bytes;
+    struct Guard {
+        smax_hip_ctx ctx;
+        smax_hip_scs_info *p;
+        bool keep = false;
+        ~Guard() {
+            if (keep) return;
+            (void)hipStreamSynchronize(ctx->stream);
+            scs_free_arrays(*p);
+        }
+    } guard{ctx, &info};
+    Scratch S(ctx);
+    int32_t *bad = nullptr;
+    int rc = S.get(ctx, &bad, 1, "scs adoption");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.chunk_ptr),
+                    sizeof(int32_t) * (n_chunks + 1), "scs chunk_ptr");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.chunk_lengths),
+                    sizeof(int32_t) * n_chunks, "scs chunk_lengths");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.col),
+                    sizeof(int32_t) * (n_elements > 0 ? n_elements : 1), "scs col");
+    rc |= dev_alloc(ctx, &info.val, val_bytes * (n_elements > 0 ? n_elements : 1), "scs val");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.perm), sizeof(int32_t) * n_rows,
+                    "scs perm");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.slot_row),
+                    sizeof(int32_t) * n_rows_padded, "scs slot_row");
+    rc |= dev_alloc(ctx, reinterpret_cast<void **>(&info.slot_len),
+                    sizeof(int32_t) * n_rows_padded, "scs slot_len");
+    if (rc) return SMAX_HIP_ERROR_OUT_OF_MEMORY;
+    const auto d2d = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    SMAX_HIP_CALL(ctx, d2d(info.chunk_ptr, chunk_ptr, sizeof(int32_t) * (n_chunks + 1)));
+    SMAX_HIP_CALL(ctx, d2d(info.chunk_lengths, chunk_lengths, sizeof(int32_t) * n_chunks));
+    if (n_elements > 0) {
+        SMAX_HIP_CALL(ctx, d2d(info.col, col, sizeof(int32_t) * n_elements));
+        SMAX_HIP_CALL(ctx, d2d(info.val, val, val_bytes * n_elements));
+    }
+    SMAX_HIP_CALL(ctx, d2d(info.perm, perm, sizeof(int32_t) * n_rows));
+    LAUNCH(k_fill_i32, 1, 1, 0, bad);
+    LAUNCH(k_check_sell_chunks, n_chunks + 1, n_chunks, C, n_elements, info.chunk_ptr,
+           info.chunk_lengths, bad);
+    int32_t h_bad = 0;
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_bad)
+        return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                    "smax_hip_scs_from_sell: chunk_ptr does not match C * chunk_lengths "
+                    "(or chunk_ptr[n_chunks] != n_elements)");
+    if (n_elements > 0)
+        LAUNCH(k_check_sell_cols, n_elements, n_elements, n_cols, info.col, bad);
+    LAUNCH(k_fill_i32, n_rows_padded, n_rows_padded, static_cast<int32_t>(n_rows),
+           info.slot_row);
+    LAUNCH(k_slot_row_from_perm, n_rows, n_rows, n_rows_padded, info.perm, info.slot_row, bad);
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_bad)
+        return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                    "smax_hip_scs_from_sell: a column index lies outside [0, n_cols) or a perm "
+                    "entry outside [0, n_rows_padded)");
+    LAUNCH(k_check_perm, n_rows, n_rows, info.perm, info.slot_row, bad);
+    switch (val_bytes) {
+    case 8:
+        LAUNCH(k_sell_slot_len<uint64_t>, n_rows_padded, n_rows_padded, C, info.chunk_ptr,
+               info.chunk_lengths, info.col, static_cast<const uint64_t *>(info.val),
+               info.slot_len);
+        break;
+    case 4:
+        LAUNCH(k_sell_slot_len<uint32_t>, n_rows_padded, n_rows_padded, C, info.chunk_ptr,
+               info.chunk_lengths, info.col, static_cast<const uint32_t *>(info.val),
+               info.slot_len);
+        break;
+    default:
+        LAUNCH(k_sell_slot_len<uint16_t>, n_rows_padded, n_rows_padded, C, info.chunk_ptr,
+               info.chunk_lengths, info.col, static_cast<const uint16_t *>(info.val),
+               info.slot_len);
+        break;
+    }
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_bad)
+        return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
+                    "smax_hip_scs_from_sell: perm maps two rows to one slot");
+    smax_hip_scs scs = new smax_hip_scs_s();
+    scs->info = info;
+    scs->from_sell = true;
+    guard.keep = true;
+    rc = build_schedule(ctx, C, sigma, n_rows_padded, n_chunks, n_elements, nnz,
+                        info.chunk_lengths, &scs->sched, info.slot_row);
+    if (!rc) rc = build_tails(ctx, scs);
+    if (rc) {
+        smax_hip_scs_destroy(ctx, scs);
+        return rc;
+    }
+    *out = scs;
+    return SMAX_HIP_SUCCESS;
+}
+
 }  // namespace smaxhip
 
 using namespace smaxhip;
 
 extern "C" {
+
+int smax_hip_scs_from_sell(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows,
+                           int64_t n_rows_padded, int64_t n_cols, int64_t n_chunks,
+                           int64_t n_elements, int64_t nnz, const int32_t *chunk_ptr,
+                           const int32_t *chunk_lengths, const int32_t *col, const void *val,
+                           int64_t val_bytes, const int32_t *perm, smax_hip_scs *out) {
+    SMAX_HIP_CHECK_CTX(ctx);
+    return build_scs_from_sell(ctx, C, sigma, n_rows, n_rows_padded, n_cols, n_chunks,
+                               n_elements, nnz, chunk_ptr, chunk_lengths, col, val, val_bytes,
+                               perm, out);
+}
 
 int smax_hip_scs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
                           int64_t nnz, const int32_t *row_ptr,
@@ -741,6 +951,9 @@ int smax_hip_scs_refresh_values(smax_hip_ctx ctx, smax_hip_scs scs,
                                 const void *crs_val) {
     SMAX_HIP_CHECK_CTX(ctx);
     SMAX_HIP_REQUIRE(ctx, scs && crs_val, "smax_hip_scs_refresh_values: null");
+    SMAX_HIP_REQUIRE(ctx, !scs->from_sell,
+                     "smax_hip_scs_refresh_values: the handle was built from SELL arrays and "
+                     "has no CRS positions; build it with smax_hip_scs_from_crs");
     const smax_hip_scs_info &in = scs->info;
     const int64_t m = in.n_elements;
     // the solvers' rounded copies follow the old values: drop them (rebuilt

@@ -1048,6 +1048,9 @@ int smax_hip_scs_apply_spmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                             int64_t k, const void *X, void *Y) {
     SMAX_HIP_CHECK_CTX(ctx);
     SMAX_HIP_REQUIRE(ctx, A, "smax_hip_scs_apply_spmm: null matrix");
+    SMAX_HIP_REQUIRE(ctx, !A->from_sell,
+                     "smax_hip_scs_apply_spmm: the matrix was built from SELL arrays, whose slot "
+                     "lengths may drop stored +0 entries; build it with smax_hip_scs_from_crs");
     const smax_hip_scs_info &in = A->info;
     int rc = check_scs_args(ctx, "smax_hip_scs_apply_spmm", value_type, in.C,
                             in.n_chunks, in.chunk_ptr, in.chunk_lengths, in.col, in.val,

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <memory>
+#include <vector>
 
 #include "../../platforms/hip/hip.hpp"
 #include "spmv_common.hpp"
@@ -39,18 +40,17 @@ inline size_t vector_bytes(FloatType t) { return t == FloatType::FLOAT64 ? 8 : 4
 
 struct DeviceState {
     PLATFORM::HIP::Stream stream;
-    DeviceBuffer a0, a1, a2, a3;  // format arrays (see initialize)
+    DeviceBuffer a0, a1, a2, a3, a4;  // format arrays (see initialize)
     DeviceBuffer x, y;
     smax_hip_crs_plan plan = nullptr;
     const void *plan_row_ptr = nullptr;  // host row_ptr the plan was built for
     ULL plan_rows = 0, plan_nnz = 0;
-    // SELL execution schedule, keyed like the CRS plan by the host array
-    smax_hip_scs_plan scs_plan = nullptr;
-    const void *scs_plan_lengths = nullptr;
-    ULL scs_plan_C = 0, scs_plan_sigma = 0, scs_plan_chunks = 0;
+    // SELL: a handle over the uploaded arrays (slot lengths, execution
+    // schedule, tail waves), rebuilt at every initialize
+    smax_hip_scs scs = nullptr;
     ~DeviceState() {
         if (plan) smax_hip_crs_plan_destroy(stream.get(), plan);
-        if (scs_plan) smax_hip_scs_plan_destroy(stream.get(), scs_plan);
+        if (scs) smax_hip_scs_destroy(stream.get(), scs);
     }
 };
 
@@ -69,28 +69,40 @@ inline int initialize(Timers *timers, KernelContext *k, Args *args, Flags *flags
     ULL x_size, y_size;
     int rc = 0;
     if (flags->is_mat_scs) {
+        // The arrays go up as registered and become a handle (copied), which
+        // derives each slot's length, the execution schedule and the tail
+        // waves from them: rebuilt here, so it always matches the values
+        // registered now.  The reference's SELL kernel never reads perm; a
+        // matrix registered without one runs with the identity.
         const SCSMatrix &A = *args->A.scs;
+        if (d.scs) smax_hip_scs_destroy(ctx, d.scs);
+        d.scs = nullptr;
         rc |= d.a0.upload(ctx, A.chunk_ptr, 4 * (A.n_chunks + 1));
         rc |= d.a1.upload(ctx, A.chunk_lengths, 4 * A.n_chunks);
         rc |= d.a2.upload(ctx, A.col, 4 * A.n_elements);
         rc |= d.a3.upload(ctx, A.val, vb * A.n_elements);
+        if (A.perm) {
+            rc |= d.a4.upload(ctx, A.perm, 4 * A.n_rows);
+        } else {
+            std::vector<int32_t> id(A.n_rows);
+            for (ULL r = 0; r < A.n_rows; ++r) id[r] = static_cast<int32_t>(r);
+            rc |= d.a4.upload(ctx, id.data(), 4 * A.n_rows);
+            if (!rc) rc |= report(ctx, smax_hip_sync(ctx), "synchronize");
+        }
         x_size = A.n_cols;
         y_size = A.n_rows_padded;
-        if (!rc && (!d.scs_plan || d.scs_plan_lengths != A.chunk_lengths ||
-                    d.scs_plan_C != A.C || d.scs_plan_sigma != A.sigma ||
-                    d.scs_plan_chunks != A.n_chunks)) {
-            if (d.scs_plan) smax_hip_scs_plan_destroy(ctx, d.scs_plan);
-            d.scs_plan = nullptr;
+        if (!rc)
             rc |= report(ctx,
-                         smax_hip_scs_plan_create(ctx, A.C, A.sigma, A.n_rows_padded,
-                                                  A.n_chunks, A.n_elements, A.nnz,
-                                                  d.a1.as<int32_t>(), &d.scs_plan),
-                         "SELL execution plan");
-            d.scs_plan_lengths = A.chunk_lengths;
-            d.scs_plan_C = A.C;
-            d.scs_plan_sigma = A.sigma;
-            d.scs_plan_chunks = A.n_chunks;
-        }
+                         smax_hip_scs_from_sell(ctx, A.C, A.sigma, A.n_rows, A.n_rows_padded,
+                                                A.n_cols, A.n_chunks, A.n_elements, A.nnz,
+                                                d.a0.as<int32_t>(), d.a1.as<int32_t>(),
+                                                d.a2.as<int32_t>(), d.a3.get(),
+                                                static_cast<int64_t>(vb), d.a4.as<int32_t>(),
+                                                &d.scs),
+                         "SELL handle");
+        // the handle holds its own copies of the big arrays
+        d.a2.release();
+        d.a3.release();
     } else if (flags->is_mat_bcrs) {
         const BCRSMatrix &A = *args->A.bcrs;
         rc |= d.a0.upload(ctx, A.row_ptr, 4 * (A.n_rows + 1));
@@ -140,14 +152,8 @@ inline int apply(Timers *timers, KernelContext *k, Args *args, Flags *flags,
     void *y = static_cast<char *>(d.y.get()) + xb * y_off;
     int st;
     if (flags->is_mat_scs) {
-        const SCSMatrix &A = *args->A.scs;
-        st = d.scs_plan
-                 ? smax_hip_scs_spmv_plan(ctx, vt, d.scs_plan, d.a0.as<int32_t>(),
-                                          d.a1.as<int32_t>(), d.a2.as<int32_t>(),
-                                          d.a3.get(), x, y)
-                 : smax_hip_scs_spmv(ctx, vt, A.C, A.n_chunks, d.a0.as<int32_t>(),
-                                     d.a1.as<int32_t>(), d.a2.as<int32_t>(),
-                                     d.a3.get(), x, y);
+        // y in SELL order (n_rows_padded entries), as the reference writes it
+        st = smax_hip_scs_apply(ctx, vt, d.scs, x, y, 0);
     } else if (flags->is_mat_bcrs) {
         const BCRSMatrix &A = *args->A.bcrs;
         st = smax_hip_bcrs_spmv(ctx, vt, A.n_rows, A.b_height, A.b_width,

@@ -423,3 +423,133 @@ def test_scs_apply_few_long_rows(ctx, vt, n, sigma, n_long):
     for d in (dx, dy, yp):
         d.free()
     s.destroy()
+
+
+# ------------------------------------------------- handles over SELL arrays
+def sell_handle(ctx, S, vb):
+    """A SpMV handle over the device arrays of handle S (as a caller that
+    registered SELL arrays would hold them)."""
+    i = S.info
+    return ctx.scs_from_sell(i.C, i.sigma, i.n_rows, i.n_rows_padded, i.n_cols, i.n_chunks,
+                             i.n_elements, i.nnz, i.chunk_ptr, i.chunk_lengths, i.col, i.val,
+                             vb, i.perm)
+
+
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 4096), (32, 64), (32, 1 << 20), (8, 16)])
+def test_scs_from_sell_equals_converted_handle(ctx, vt, C_, sigma):
+    """A handle built from SELL arrays derives the converted handle's slot
+    lengths, rows and tail waves from the arrays alone (no stored entry of
+    this matrix is +0.0), and its SpMV equals it bit for bit in both output
+    orders."""
+    A = matgen.lomax_band(30000, half=3750)
+    val, x = values_for(vt, A)
+    S = gpu_scs(ctx, A, C_, sigma, val)
+    T = sell_handle(ctx, S, val.dtype.itemsize)
+    a, b = S.arrays(val.dtype), T.arrays(val.dtype)
+    assert b["src_index"] is None
+    for f in ("chunk_ptr", "chunk_lengths", "col", "perm", "slot_len"):
+        assert np.array_equal(a[f], b[f]), f
+    assert np.array_equal(a["val"].view(np.uint8), b["val"].view(np.uint8))
+    real = a["slot_row"] < A.n_rows
+    assert np.array_equal(a["slot_row"][real], b["slot_row"][real])
+    assert (b["slot_row"][~real] >= A.n_rows).all()
+    assert S.tails() == T.tails()
+    dx = ctx.upload(x)
+    for unperm, n in ((False, S.info.n_rows_padded), (True, A.n_rows)):
+        y1, y2 = ctx.zeros(n, x.dtype), ctx.zeros(n, x.dtype)
+        S.apply(vt, dx, y1, unpermuted=unperm)
+        T.apply(vt, dx, y2, unpermuted=unperm)
+        assert_same_values(ctx.download(y2), ctx.download(y1), f"unpermuted={unperm}")
+        y1.free()
+        y2.free()
+    dx.free()
+    T.destroy()
+    S.destroy()
+
+
+@pytest.mark.parametrize("x0", [1.0, -1.0, -0.0, np.inf, np.nan])
+def test_scs_from_sell_trailing_zero_entries(ctx, x0):
+    """Rows whose last stored entries sit at column 0 (CRS order need not be
+    sorted): a trailing +0.0 there is the padding term exactly and may be cut
+    from the slot's length; a trailing -0.0 may not (fma(-0, x, -0) = -0 where
+    the padding term gives +0).  The SpMV still equals the reference's full
+    padded chain bit for bit for every x[0], also for rows of -0.0 only."""
+    rows = [[(5, 1.5), (0, 0.0)], [(0, -0.0)], [(0, 0.0)], [(3, -2.0), (0, -0.0)],
+            [], [(7, 0.25), (0, 0.0), (0, 0.0)], [(2, -0.0), (0, 0.0)],
+            [(1, -1.0)], [(0, -0.0), (0, 0.0)]] * 23
+    n = len(rows)
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.array([c for r in rows for c, _ in r], np.int32)
+    val = np.array([v for r in rows for _, v in r], np.float64)
+    x = matgen.uniform_f64(5, n)
+    x[0] = x0
+    for C_, sigma in ((8, 1), (4, 8), (32, 1)):
+        want_s = oracle.crs_to_scs(rp, col, val, C_, sigma)
+        want = oracle.scs_spmv(C_, want_s["n_chunks"], want_s["chunk_ptr"],
+                               want_s["chunk_lengths"], want_s["col"], want_s["val"], x)
+        d = {k: ctx.upload(want_s[k]) for k in ("chunk_ptr", "chunk_lengths", "col", "val",
+                                               "perm")}
+        T = ctx.scs_from_sell(C_, sigma, n, want_s["n_rows_padded"], n, want_s["n_chunks"],
+                              want_s["n_elements"], len(col), d["chunk_ptr"],
+                              d["chunk_lengths"], d["col"], d["val"], 8, d["perm"])
+        dx = ctx.upload(x)
+        dy = ctx.zeros(want_s["n_rows_padded"], np.float64)
+        T.apply(capi.F64, dx, dy)
+        assert_same_values(ctx.download(dy), want, f"C={C_} sigma={sigma}")
+        T.destroy()
+        for a in (*d.values(), dx, dy):
+            a.free()
+
+
+def test_scs_from_sell_refusals(ctx):
+    """Arrays that do not form a SELL layout are refused before any kernel
+    reads through them, and a handle over SELL arrays refuses what needs CRS
+    positions."""
+    A = matgen.lomax_band(5000, half=600)
+    S = gpu_scs(ctx, A, 32, 64)
+    a = S.arrays(np.float64)
+    i = S.info
+    lib = ctx.lib
+
+    def build(**over):
+        arrs = {k: a[k].copy() for k in ("chunk_ptr", "chunk_lengths", "col", "val", "perm")}
+        arrs.update(over)
+        d = {k: ctx.upload(v) for k, v in arrs.items()}
+        h = capi.vp()
+        st = lib.smax_hip_scs_from_sell(ctx.handle, i.C, i.sigma, i.n_rows, i.n_rows_padded,
+                                        i.n_cols, i.n_chunks, i.n_elements, i.nnz,
+                                        d["chunk_ptr"].ptr, d["chunk_lengths"].ptr,
+                                        d["col"].ptr, d["val"].ptr, 8, d["perm"].ptr,
+                                        capi.C.byref(h))
+        for v in d.values():
+            v.free()
+        return st, h.value
+
+    bad_col = a["col"].copy()
+    bad_col[len(bad_col) // 2] = A.n_cols
+    bad_ptr = a["chunk_ptr"].copy()
+    bad_ptr[3] += 1
+    dup = a["perm"].copy()
+    dup[1] = dup[0]
+    oob = a["perm"].copy()
+    oob[7] = i.n_rows_padded
+    for name, over in (("column", {"col": bad_col}), ("chunk_ptr", {"chunk_ptr": bad_ptr}),
+                       ("perm duplicate", {"perm": dup}), ("perm range", {"perm": oob})):
+        st, h = build(**over)
+        assert st == capi.ERROR_INVALID_VALUE and h is None, name
+    st, h = build()
+    assert st == capi.SUCCESS
+    T = capi.Scs(ctx, h)
+    dv = ctx.zeros(A.nnz, np.float64)
+    G = ctx.zeros(A.n_rows * 4, np.float64)
+    with pytest.raises(capi.SmaxHipError):
+        T.refresh_values(dv)
+    with pytest.raises(capi.SmaxHipError):
+        T.sddmm(capi.F64, 4, G, G, dv)
+    with pytest.raises(capi.SmaxHipError):
+        T.apply_spmm(capi.F64, 4, G, G)
+    for d in (dv, G):
+        d.free()
+    T.destroy()
+    S.destroy()
