@@ -45,6 +45,7 @@ struct DeviceState {
     smax_hip_crs_plan plan = nullptr;
     const void *plan_row_ptr = nullptr;  // host row_ptr the plan was built for
     ULL plan_rows = 0, plan_nnz = 0;
+    bool use_plan = false;               // CRS: row blocks chosen at initialize
     // SELL: a handle over the uploaded arrays (slot lengths, execution
     // schedule, tail waves), rebuilt at every initialize
     smax_hip_scs scs = nullptr;
@@ -117,7 +118,22 @@ inline int initialize(Timers *timers, KernelContext *k, Args *args, Flags *flags
         rc |= d.a2.upload(ctx, A.val, vb * A.nnz);
         x_size = A.n_cols;
         y_size = A.n_rows;
-        const bool blocked = flags->kernel_type != SpMVType::naive_thread_per_row;
+        // Both CRS schedules give every row one lane's fma chain (bit-identical
+        // results); they differ in how the entries reach the lanes.  The warp
+        // types select the row blocks; naive_thread_per_row keeps the direct
+        // lane-per-row loads for rows of even length (C1, 512^2 Laplacian:
+        // 3.4 us against 5.8) and takes the row blocks when the longest row
+        // exceeds 4x the mean (C2, Lomax rows: 0.78 ms against 1.37).
+        bool blocked = flags->kernel_type != SpMVType::naive_thread_per_row;
+        if (!blocked && A.n_rows > 0) {
+            const int32_t *rp = static_cast<const int32_t *>(A.row_ptr);
+            int64_t longest = 0;
+            for (ULL r = 0; r < A.n_rows; ++r)
+                longest = std::max<int64_t>(longest, static_cast<int64_t>(rp[r + 1]) - rp[r]);
+            blocked = static_cast<double>(longest) > 4.0 * static_cast<double>(A.nnz) /
+                                                         static_cast<double>(A.n_rows);
+        }
+        d.use_plan = blocked;
         if (!rc && blocked &&
             (!d.plan || d.plan_row_ptr != A.row_ptr || d.plan_rows != A.n_rows ||
              d.plan_nnz != A.nnz)) {
@@ -164,7 +180,7 @@ inline int apply(Timers *timers, KernelContext *k, Args *args, Flags *flags,
         const CRSMatrix &A = *args->A.crs;
         st = smax_hip_crs_spmv(ctx, vt, A.n_rows, A.n_cols, A.nnz,
                                d.a0.as<int32_t>(), d.a1.as<int32_t>(), d.a2.get(),
-                               x, y, d.plan);
+                               x, y, d.use_plan ? d.plan : nullptr);
     }
     return report(ctx, st, "SpMV launch");
 }
