@@ -63,6 +63,10 @@ class Kernel {
     virtual int set_mat_scs(bool) { return unsupported("set_mat_scs"); }
     virtual int set_mat_bcrs(bool) { return unsupported("set_mat_bcrs"); }
     virtual int set_block_column_major(bool) { return unsupported("set_block_column_major"); }
+    // SELL SpMV writes y in original row order (n_rows entries) instead of the
+    // SELL order of the reference (n_rows_padded entries).  No reference
+    // counterpart: the reference leaves y permuted for the caller to undo.
+    virtual int set_scs_original_order(bool) { return unsupported("set_scs_original_order"); }
     virtual int set_kernel_implementation(SpMVType) {
         return unsupported("set_kernel_implementation(SpMVType)");
     }

@@ -89,6 +89,7 @@ class SpMVKernel : public Kernel {
     int set_mat_scs(bool f) override { flags.is_mat_scs = f; return 0; }
     int set_mat_bcrs(bool f) override { flags.is_mat_bcrs = f; return 0; }
     int set_block_column_major(bool f) override { flags.is_block_column_major = f; return 0; }
+    int set_scs_original_order(bool f) override { flags.scs_original_order = f; return 0; }
 
     int swap_operands() override {
         std::swap(args.x, args.y);

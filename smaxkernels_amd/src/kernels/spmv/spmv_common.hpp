@@ -15,16 +15,17 @@ struct Flags {
     bool is_mat_scs = false;
     bool is_mat_bcrs = false;
     bool is_block_column_major = false;
+    bool scs_original_order = false;  // SELL y in original row order
 };
 
 // Entries of x the product reads and of y it writes, per format
 // (spmv_cuda_core.cuh:34-110 sizes the device copies the same way): CRS
-// n_cols / n_rows, SELL n_cols / n_chunks * C, BCRS n_cols * w_pad /
-// n_rows * h_pad.
+// n_cols / n_rows, SELL n_cols / n_chunks * C (n_rows in original row
+// order), BCRS n_cols * w_pad / n_rows * h_pad.
 inline void operand_extents(const Args &a, const Flags &f, ULL &x_need, ULL &y_need) {
     if (f.is_mat_scs) {
         x_need = a.A.scs->n_cols;
-        y_need = a.A.scs->n_chunks * a.A.scs->C;
+        y_need = f.scs_original_order ? a.A.scs->n_rows : a.A.scs->n_chunks * a.A.scs->C;
     } else if (f.is_mat_bcrs) {
         x_need = a.A.bcrs->n_cols * a.A.bcrs->width_pad;
         y_need = a.A.bcrs->n_rows * a.A.bcrs->height_pad;

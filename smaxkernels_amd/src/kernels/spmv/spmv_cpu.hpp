@@ -150,10 +150,22 @@ struct SpMVApply {
         A *y = static_cast<A *>(a.y.val) + y_off;
         if (fl.is_mat_scs) {
             const SCSMatrix &m = *a.A.scs;
+            std::vector<A> tmp;
+            A *ys = y;
+            if (fl.scs_original_order) {
+                tmp.resize(m.n_chunks * m.C);
+                ys = tmp.data();
+            }
             scs_spmv<IT, V, A>(m.C, m.n_chunks, static_cast<const IT *>(m.chunk_ptr),
                                static_cast<const IT *>(m.chunk_lengths),
                                static_cast<const IT *>(m.col), static_cast<const V *>(m.val), x,
-                               y);
+                               ys);
+            if (fl.scs_original_order) {
+                // row r sits at slot perm[r] (convert_formats.hpp:81-86); no perm: r
+                const IT *perm = static_cast<const IT *>(m.perm);
+                for (ULL r = 0; r < m.n_rows; ++r)
+                    y[r] = tmp[perm ? static_cast<ULL>(perm[r]) : r];
+            }
         } else if (fl.is_mat_bcrs) {
             const BCRSMatrix &m = *a.A.bcrs;
             bcrs_spmv<IT, V, A>(m, fl.is_block_column_major, static_cast<const IT *>(m.col),

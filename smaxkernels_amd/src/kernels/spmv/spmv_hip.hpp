@@ -91,7 +91,7 @@ inline int initialize(Timers *timers, KernelContext *k, Args *args, Flags *flags
             if (!rc) rc |= report(ctx, smax_hip_sync(ctx), "synchronize");
         }
         x_size = A.n_cols;
-        y_size = A.n_rows_padded;
+        y_size = flags->scs_original_order ? A.n_rows : A.n_rows_padded;
         if (!rc)
             rc |= report(ctx,
                          smax_hip_scs_from_sell(ctx, A.C, A.sigma, A.n_rows, A.n_rows_padded,
@@ -168,8 +168,9 @@ inline int apply(Timers *timers, KernelContext *k, Args *args, Flags *flags,
     void *y = static_cast<char *>(d.y.get()) + xb * y_off;
     int st;
     if (flags->is_mat_scs) {
-        // y in SELL order (n_rows_padded entries), as the reference writes it
-        st = smax_hip_scs_apply(ctx, vt, d.scs, x, y, 0);
+        // y in SELL order (n_rows_padded entries), as the reference writes it,
+        // or in original row order (n_rows entries)
+        st = smax_hip_scs_apply(ctx, vt, d.scs, x, y, flags->scs_original_order ? 1 : 0);
     } else if (flags->is_mat_bcrs) {
         const BCRSMatrix &A = *args->A.bcrs;
         st = smax_hip_bcrs_spmv(ctx, vt, A.n_rows, A.b_height, A.b_width,

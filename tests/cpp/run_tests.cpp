@@ -163,6 +163,25 @@ TEST(host, cpu_spmv_scs_kat) {
     close(y.data(), want.data(), 4, "scs_y");
 }
 
+TEST(host, cpu_spmv_scs_original_order) {
+    // the SELL KAT above with y in original row order: y[r] = y_scs[perm[r]]
+    SMAX::Interface smax;
+    std::vector<int> chunk_ptr = {0, 4, 6}, chunk_len = {2, 1}, col = {0, 0, 1, 2, 1, 0},
+                     perm = {0, 2, 1};
+    std::vector<double> val = {1.1, 3.1, 1.2, 3.3, 2.2, 0.0};
+    std::vector<double> x(3, 1.0), y(3, -1.0), want = {2.3, 2.2, 6.4};
+    smax.register_kernel("scs", KernelType::SPMV, PlatformType::CPU);
+    auto *k = smax.kernel("scs");
+    k->set_mat_scs(true);
+    check(k->set_scs_original_order(true) == 0, "flag");
+    k->register_A(2, 4, 3, 4, 3, 2, 6, 5, chunk_ptr.data(), chunk_len.data(), col.data(),
+                  val.data(), perm.data());
+    k->register_B(3, x.data());
+    k->register_C(3, y.data());  // n_rows entries suffice in original order
+    check(k->run() == 0, "run");
+    close(y.data(), want.data(), 3, "scs_y original order");
+}
+
 TEST(host, cpu_bcrs_kats) {  // spmv_bcrs_1_1_test_3.hpp, both block layouts
     std::vector<int> col = {0, 1, 1}, rp = {0, 2, 3};
     // block (ix, jx): column-major at jx*2 + ix, row-major at ix*2 + jx
@@ -732,6 +751,26 @@ TEST(gpu, sell_cpu_and_hip_platforms_agree_bitwise) {
                 same &= (std::isnan(y[0][i]) && std::isnan(y[1][i])) ||
                         std::memcmp(&y[0][i], &y[1][i], sizeof(double)) == 0;
             for (unsigned long long i = n_rows; i < n_rows_padded; ++i) same &= y[1][i] == -7.0;
+            // the same products written in original row order: row r of y is
+            // entry perm[r] of the SELL-ordered one, on both platforms
+            for (int p = 0; p < 2; ++p) {
+                SMAX::Interface smax;
+                std::vector<double> x(m.x), z(n_rows, -7.0);
+                smax.register_kernel("s", KernelType::SPMV, p ? PlatformType::HIP : PlatformType::CPU);
+                auto *k = smax.kernel("s");
+                k->set_mat_scs(true);
+                check(k->set_scs_original_order(true) == 0, "original order flag");
+                k->register_A(static_cast<unsigned long long>(cs.first),
+                              static_cast<unsigned long long>(cs.second), n_rows, n_rows_padded,
+                              n_cols, n_chunks, n_elements, nnz, chunk_ptr, chunk_lengths, col,
+                              val, perm);
+                k->register_B(n_cols, x.data());
+                k->register_C(n_rows, z.data());
+                check(k->run() == 0, "sell run, original order");
+                for (unsigned long long r = 0; r < n_rows; ++r)
+                    same &= (std::isnan(z[r]) && std::isnan(y[0][perm[r]])) ||
+                            std::memcmp(&z[r], &y[0][perm[r]], sizeof(double)) == 0;
+            }
             check(same, "SELL C=" + std::to_string(cs.first) + " sigma=" + std::to_string(cs.second));
             delete[] chunk_ptr;
             delete[] chunk_lengths;
