@@ -7,15 +7,21 @@
 //
 //  * sequential: one lane per row reading straight from HBM (the reference's
 //    CUDA schedule, spmv_cuda_crs_impl.cu:5-22);
-//  * blocked: a plan bins rows by length into row blocks of at most 256 rows
-//    whose first nonzeros fall in one 1024-entry window; rows of 512 or more
-//    nonzeros form a block of their own.  Each workgroup streams its block's
-//    values and column indices through LDS in coalesced 1024-entry tiles
+//  * blocked: a plan cuts the rows into row blocks of at most T rows whose
+//    first nonzeros fall in one W-entry window; rows of 512 or more nonzeros
+//    form a block of their own.  Each workgroup of T threads streams its
+//    block's values and column indices through LDS in coalesced W-entry tiles
 //    (gathering x on the way), and each row's lane then runs its fma chain out
 //    of LDS.  Short rows are thereby reduced lane-per-row while the loads stay
 //    fully coalesced; a long row is streamed by its whole workgroup.  Row
 //    blocks are dealt to the XCDs in contiguous ranges (xcd_block), so the x
-//    band of neighbouring blocks is shared in one L2.
+//    band of neighbouring blocks is shared in one L2.  Two geometries:
+//    T = 256, W = 1024 for rows of even length, and single-wave blocks,
+//    T = 64, W = 512, when the longest row exceeds 4x the mean: a wave then
+//    waits at its own tiles' barriers only (4M-row band matrix, FP64:
+//    0.692 ms against 0.766; BF16 values 0.512 against 0.692; the 512^2
+//    Laplacian and a 27-point stencil run 6.0 / 25.5 us with the wide blocks
+//    against 6.5 / 28.2 us).
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -26,6 +32,7 @@
 
 struct smax_hip_crs_plan_s {
     int64_t n_rows = 0, nnz = 0, n_blocks = 0;
+    int32_t block_threads = 256;     // row-block geometry: 256 (wide) or 64 (narrow)
     int32_t *block_start = nullptr;  // n_blocks + 1 row indices
     int32_t *row_order = nullptr;    // row-major SpMM execution order (n_rows)
     int64_t spmm_lds = 0;            // dynamic LDS per SpMM workgroup (occupancy cap)
@@ -34,9 +41,11 @@ struct smax_hip_crs_plan_s {
 namespace smaxhip {
 
 constexpr int kCrsThreads = 256;
-constexpr int kCrsTile = 1024;       // nonzeros staged per LDS tile
-constexpr int kCrsWindow = 1024;     // block-start window in nonzeros
 constexpr int kCrsLongRow = 512;     // rows at least this long stream alone
+// row-block geometries: threads per block (= rows at most) and the tile and
+// block-start window in nonzeros
+constexpr int kBlkWideThreads = 256, kBlkWideTile = 1024;
+constexpr int kBlkNarrowThreads = 64, kBlkNarrowTile = 512;
 
 // ------------------------------------------------------------ sequential
 template <int VT>
@@ -72,8 +81,8 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmv_lane(
 }
 
 // ------------------------------------------------------------ blocked
-template <int VT>
-__global__ __launch_bounds__(kCrsThreads) void crs_spmv_blocked(
+template <int VT, int NT, int TILE>
+__global__ __launch_bounds__(NT) void crs_spmv_blocked(
     const int32_t *__restrict__ block_start,
     const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
     const typename vt<VT>::val_t *__restrict__ val,
@@ -81,8 +90,8 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmv_blocked(
     typename vt<VT>::vec_t *__restrict__ y) {
     using T = vt<VT>;
     using A = typename T::vec_t;
-    __shared__ A sv[kCrsTile];
-    __shared__ A sx[kCrsTile];
+    __shared__ A sv[TILE];
+    __shared__ A sx[TILE];
     const int tid = threadIdx.x;
     const int64_t b = xcd_block(blockIdx.x, gridDim.x);
     const int64_t r0 = block_start[b], r1 = block_start[b + 1];
@@ -91,14 +100,14 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmv_blocked(
     const bool has = row < r1;
     const int64_t rs = has ? row_ptr[row] : 0, re = has ? row_ptr[row + 1] : 0;
     A acc = 0;
-    constexpr int PER = kCrsTile / kCrsThreads;
-    for (int64_t t0 = s; t0 < e; t0 += kCrsTile) {
-        const int64_t nt = e - t0 < kCrsTile ? e - t0 : kCrsTile;
+    constexpr int PER = TILE / NT;
+    for (int64_t t0 = s; t0 < e; t0 += TILE) {
+        const int64_t nt = e - t0 < TILE ? e - t0 : TILE;
         int32_t c[PER];
         typename T::val_t v[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int64_t k = tid + u * kCrsThreads;
+            const int64_t k = tid + u * NT;
             if (k < nt) {
                 c[u] = ld_stream(col + t0 + k);
                 v[u] = ld_stream(val + t0 + k);
@@ -106,7 +115,7 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmv_blocked(
         }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int64_t k = tid + u * kCrsThreads;
+            const int64_t k = tid + u * NT;
             if (k < nt) {
                 sx[k] = x[c[u]];
                 sv[k] = T::widen(v[u]);
@@ -124,18 +133,17 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmv_blocked(
     if (has) y[row] = acc;
 }
 
-// A new block starts at row r when r is a multiple of 256, when r's first
-// nonzero enters a new 1024-entry window, or when r or r-1 is a long row.
-__global__ void crs_block_flags(int64_t n_rows,
-                                const int32_t *__restrict__ row_ptr,
-                                int32_t *__restrict__ flag) {
+// A new block starts at row r when r is a multiple of nt (the block's
+// threads), when r's first nonzero enters a new window of `window` entries,
+// or when r or r-1 is a long row.
+__global__ void crs_block_flags(int64_t n_rows, const int32_t *__restrict__ row_ptr,
+                                int32_t *__restrict__ flag, int32_t nt, int32_t window) {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
     int32_t f = 1;
-    if (r % kCrsThreads != 0) {
+    if (r % nt != 0) {
         const int32_t a = row_ptr[r - 1], b = row_ptr[r], c = row_ptr[r + 1];
-        f = (a / kCrsWindow != b / kCrsWindow) || (b - a >= kCrsLongRow) ||
-            (c - b >= kCrsLongRow);
+        f = (a / window != b / window) || (b - a >= kCrsLongRow) || (c - b >= kCrsLongRow);
     }
     flag[r] = f;
 }
@@ -326,13 +334,9 @@ constexpr int64_t kSpmmWindow = 65536;
 constexpr int64_t kSpmmCapLds = 53 * 1024;
 
 static int build_row_order(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
-                           const int32_t *row_ptr, int32_t **out, int64_t *n_long) {
+                           const std::vector<int32_t> &rp, int32_t **out, int64_t *n_long) {
     *out = nullptr;
     *n_long = 0;
-    std::vector<int32_t> rp(static_cast<size_t>(n_rows + 1));
-    SMAX_HIP_CALL(ctx, hipMemcpyAsync(rp.data(), row_ptr, sizeof(int32_t) * (n_rows + 1),
-                                      hipMemcpyDeviceToHost, ctx->stream));
-    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
     const double mean = static_cast<double>(nnz) / static_cast<double>(n_rows);
     const int64_t thr =
         std::max<int64_t>(128, static_cast<int64_t>(std::ceil(8 * mean)));
@@ -541,6 +545,18 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     SMAX_HIP_REQUIRE(ctx, n_rows > 0 && n_rows < INT32_MAX && nnz >= 0 &&
                               nnz <= INT32_MAX && row_ptr,
                      "smax_hip_crs_plan_create: bad arguments");
+    // the row lengths decide the block geometry and the SpMM row order
+    std::vector<int32_t> rp(static_cast<size_t>(n_rows + 1));
+    SMAX_HIP_CALL(ctx, hipMemcpyAsync(rp.data(), row_ptr, sizeof(int32_t) * (n_rows + 1),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SMAX_HIP_CALL(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t longest = 0;
+    for (int64_t r = 0; r < n_rows; ++r)
+        longest = std::max<int64_t>(longest, static_cast<int64_t>(rp[r + 1]) - rp[r]);
+    const bool narrow = static_cast<double>(longest) >
+                        4.0 * static_cast<double>(nnz) / static_cast<double>(n_rows);
+    const int32_t nt = narrow ? kBlkNarrowThreads : kBlkWideThreads;
+    const int32_t window = narrow ? kBlkNarrowTile : kBlkWideTile;
     int32_t *flag = nullptr, *tmp = nullptr, *starts = nullptr;
     int rc = dev_alloc(ctx, reinterpret_cast<void **>(&flag),
                        sizeof(int32_t) * n_rows, "crs plan");
@@ -553,7 +569,7 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     int64_t n_blocks = 0;
     if (!rc) {
         hipLaunchKernelGGL(crs_block_flags, grid_for(n_rows, 256), dim3(256), 0,
-                           ctx->stream, n_rows, row_ptr, flag);
+                           ctx->stream, n_rows, row_ptr, flag, nt, window);
         rc = check_launch(ctx, "crs_block_flags");
     }
     // flag -> inclusive positions (in place); tmp is the scan scratch.
@@ -573,7 +589,7 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
                        sizeof(int32_t) * n_rows, "crs plan");
         if (!rc) {
             hipLaunchKernelGGL(crs_block_flags, grid_for(n_rows, 256), dim3(256),
-                               0, ctx->stream, n_rows, row_ptr, flag2);
+                               0, ctx->stream, n_rows, row_ptr, flag2, nt, window);
             rc = check_launch(ctx, "crs_block_flags");
         }
         if (!rc) {
@@ -588,7 +604,7 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     dev_free(tmp);
     int32_t *order = nullptr;
     int64_t n_long = 0;
-    if (!rc) rc = build_row_order(ctx, n_rows, nnz, row_ptr, &order, &n_long);
+    if (!rc) rc = build_row_order(ctx, n_rows, nnz, rp, &order, &n_long);
     if (rc) {
         dev_free(starts);
         dev_free(order);
@@ -598,6 +614,7 @@ int smax_hip_crs_plan_create(smax_hip_ctx ctx, int64_t n_rows, int64_t nnz,
     p->n_rows = n_rows;
     p->nnz = nnz;
     p->n_blocks = n_blocks;
+    p->block_threads = nt;
     p->block_start = starts;
     p->row_order = order;
     // With long rows leading, three workgroups per CU (53 KB of LDS each out
@@ -642,10 +659,15 @@ int smax_hip_crs_spmv(smax_hip_ctx ctx, int value_type, int64_t n_rows,
         auto xv = static_cast<const typename T::vec_t *>(x);
         auto yv = static_cast<typename T::vec_t *>(y);
         if (plan) {
-            hipLaunchKernelGGL((crs_spmv_blocked<VT>),
-                               dim3(static_cast<unsigned>(plan->n_blocks)),
-                               dim3(kCrsThreads), 0, ctx->stream,
-                               plan->block_start, row_ptr, col, v, xv, yv);
+            const dim3 g(static_cast<unsigned>(plan->n_blocks));
+            if (plan->block_threads == kBlkNarrowThreads)
+                hipLaunchKernelGGL((crs_spmv_blocked<VT, kBlkNarrowThreads, kBlkNarrowTile>), g,
+                                   dim3(kBlkNarrowThreads), 0, ctx->stream, plan->block_start,
+                                   row_ptr, col, v, xv, yv);
+            else
+                hipLaunchKernelGGL((crs_spmv_blocked<VT, kBlkWideThreads, kBlkWideTile>), g,
+                                   dim3(kBlkWideThreads), 0, ctx->stream, plan->block_start,
+                                   row_ptr, col, v, xv, yv);
             SMAX_HIP_LAUNCHED(ctx, "crs_spmv_blocked");
         } else {
             hipLaunchKernelGGL((crs_spmv_lane<VT>), grid_for(n_rows, kCrsThreads),
