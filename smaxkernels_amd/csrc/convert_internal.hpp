@@ -15,6 +15,9 @@ struct smax_hip_scs_sched {
     // Dynamic LDS reserved by the C = 64 SpMV launch to cap its occupancy
     // (0 = no cap).
     int64_t lds_cap_bytes = 0;
+    // Longest chunk (-1: unknown).  Layouts whose chunks all fit one short
+    // group run the C = 64 SpMV's single-group chains.
+    int32_t max_chunk_len = -1;
 };
 
 // Tail waves of a layout whose chunks mix rows of very different lengths

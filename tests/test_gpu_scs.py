@@ -228,6 +228,56 @@ def test_scs_apply_matches_full_chunk_chains(ctx, vt, C_, sigma, x0):
     s.destroy()
 
 
+def short_rows(n, seed, max_len=8, empty=0.2):
+    """Rows of 0..max_len entries: SELL-64 chunks of at most max_len."""
+    rng = np.random.default_rng(seed)
+    L = rng.integers(0, max_len + 1, n)
+    L[rng.random(n) < empty] = 0
+    rp = np.concatenate([[0], np.cumsum(L)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in L]
+                         ).astype(np.int32)
+    return Csr(n, n, rp, col, rng.uniform(-1, 1, rp[-1]))
+
+
+@pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
+@pytest.mark.parametrize("sigma", [1, 64, 1 << 20])
+@pytest.mark.parametrize("x0", [0.5, -0.0, np.inf, np.nan])
+def test_scs_apply_short_chunks(ctx, vt, sigma, x0):
+    """Layouts whose chunks hold at most 8 entries per slot run single-group
+    chains; the handle, a handle over the same SELL arrays and the plan path
+    must all equal the full padded chains bit for bit (x[0] decides the
+    padding terms)."""
+    A = short_rows(9001, 23)
+    val, x = values_for(vt, A)
+    x = x.copy()
+    x[0] = x0
+    s = gpu_scs(ctx, A, 64, sigma, val)
+    want_s = oracle.crs_to_scs(A.row_ptr, A.col, val, 64, sigma)
+    assert want_s["chunk_lengths"].max() <= 8
+    want = oracle.scs_spmv(64, want_s["n_chunks"], want_s["chunk_ptr"],
+                           want_s["chunk_lengths"], want_s["col"], want_s["val"], x)
+    dx = ctx.upload(x)
+    dy = ctx.zeros(s.info.n_rows_padded, x.dtype)
+    s.apply(vt, dx, dy)
+    assert_same_values(ctx.download(dy), want, "apply, permuted")
+    dy2 = ctx.zeros(A.n_rows, x.dtype)
+    s.apply(vt, dx, dy2, unpermuted=True)
+    assert_same_values(ctx.download(dy2), want[want_s["perm"]], "apply, original order")
+    h = sell_handle(ctx, s, val.dtype.itemsize)
+    h.apply(vt, dx, dy)
+    assert_same_values(ctx.download(dy), want, "handle over SELL arrays")
+    h.destroy()
+    i = s.info
+    plan = ctx.scs_plan(64, sigma, i.n_rows_padded, i.n_chunks, i.n_elements, A.nnz,
+                        i.chunk_lengths)
+    ctx.scs_spmv_plan(vt, plan, i.chunk_ptr, i.chunk_lengths, i.col, i.val, dx, dy)
+    assert_same_values(ctx.download(dy), want, "plan")
+    ctx.scs_plan_destroy(plan)
+    for d in (dx, dy, dy2):
+        d.free()
+    s.destroy()
+
+
 @pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 256), (64, 1 << 20), (32, 512), (8, 1)])
 def test_scs_spmv_plan_on_caller_arrays(ctx, C_, sigma):
     """The plan path (schedule only, caller-owned arrays) equals the raw SELL
@@ -313,7 +363,14 @@ def test_scs_spmv_config1_poisson(ctx):
     assert s.info.n_elements == 1309696  # reference conversion, C=64 sigma=1
     dx, dy = ctx.upload(x), ctx.zeros(A.n_rows, np.float64)
     s.spmv(capi.F64, dx, dy, unpermuted=True)
-    assert_bitwise(ctx.download(dy), oracle.crs_spmv(A.row_ptr, A.col, A.val, x))
+    want = oracle.crs_spmv(A.row_ptr, A.col, A.val, x)
+    assert_bitwise(ctx.download(dy), want)
+    # the handle runs this layout (chunks of at most 5) with single-group chains
+    dy2 = ctx.zeros(A.n_rows, np.float64)
+    s.apply(capi.F64, dx, dy2, unpermuted=True)
+    assert_bitwise(ctx.download(dy2), want, "apply")
+    for d in (dx, dy, dy2):
+        d.free()
     s.destroy()
 
 
