@@ -190,9 +190,10 @@ __device__ __forceinline__ void load_block(const typename vt<VT>::val_t *__restr
 // Register-blocked SpMM for compile-time block shapes: thread (r, l) keeps
 // the BH outputs of block row r in columns v = l*Q .. l*Q+Q-1, so each X row
 // of a block is loaded once for all BH rows (instead of once per row), and
-// the BH*Q chains give independent fmas.  Q > 1 (16-byte runs: k a multiple
-// of Q, X and Y 16-byte aligned) reads each lane's part of an X row with one
-// 16-byte load.  Blocks go in groups of G: the G column indices, then all G
+// the BH*Q chains give independent fmas.  Q = 16 bytes' worth (16-byte runs:
+// k a multiple of Q, X and Y 16-byte aligned) reads each lane's part of an X
+// row with one 16-byte load; Q = 2 with 4-byte entries takes column pairs
+// (no alignment needed).  Blocks go in groups of G: the G column indices, then all G
 // blocks' values and X entries are requested before any fma, so a row of
 // G blocks costs two memory round trips.  Per output the fma order is still
 // the reference's (block j, then jx).
@@ -266,7 +267,13 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
     // size is a multiple of 16 bytes and val is aligned)
     constexpr bool kBlock16 = (BH * BW * sizeof(typename T::val_t)) % 16 == 0;
     const bool dense = h_pad == BH && w_pad == BW && (!kBlock16 || aligned16(val, val));
-    const dim3 g = grid_for(n_block_rows * (runs ? k / Q : k), kBThreads);
+    // otherwise, with 4-byte X entries and an even k >= 8, each lane takes two
+    // columns: every lane loads its block row's blocks whole, so half as many
+    // lanes load half as many copies (96^3 block matrix, k = 8: BF16 values
+    // 0.155 -> 0.151 ms, FP32 0.198 -> 0.183 ms)
+    const bool pairs = !runs && dense && sizeof(A) == 4 && k % 2 == 0 && k >= 8;
+    const int64_t lanes = runs ? k / Q : pairs ? k / 2 : k;  // per block row
+    const dim3 g = grid_for(n_block_rows * lanes, kBThreads);
     // block groups: 8 when a single thread holds the whole block row (k = 1),
     // else 2 (measured on the 96^3 block matrix: k = 1 0.108 -> 0.058 ms,
     // k = 8 0.173 -> 0.160 ms, k = 64 0.663 -> 0.649 ms in BF16).  Padded
@@ -285,6 +292,8 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
     if (runs && dense && cm) { SMAX_BCRS_REG_LAUNCH(Q, true, true, 2); }
     else if (runs && dense) { SMAX_BCRS_REG_LAUNCH(Q, true, false, 2); }
     else if (runs) { SMAX_BCRS_REG_LAUNCH(Q, false, false, 2); }
+    else if (pairs && cm) { SMAX_BCRS_REG_LAUNCH(2, true, true, 2); }
+    else if (pairs) { SMAX_BCRS_REG_LAUNCH(2, true, false, 2); }
     else if (dense && cm) { SMAX_BCRS_REG_G(true, true, 8); }
     else if (dense) { SMAX_BCRS_REG_G(true, false, 8); }
     else { SMAX_BCRS_REG_G(false, false, kPaddedGroup); }

@@ -97,9 +97,10 @@ def test_block7_conversion_spmv_spmm(ctx, vt, bh, bw, hp, wp, cm):
                           want["row_ptr"], want["val"], cm, x)
     rows = (np.arange(i.n_block_rows * hp) % hp) < bh
     assert_bitwise(ctx.download(dy)[rows], yw[rows], "bcrs spmv")
-    # shift 1: X and Y one entry past a 16-byte boundary (one column per lane
-    # instead of 16-byte runs; the same chains)
-    for k, shift in ((1, 0), (6, 0), (8, 0), (8, 1), (64, 0)):
+    # shift 1: X and Y one entry past a 16-byte boundary (no 16-byte runs; the
+    # same chains).  k = 8, 10 and misaligned 16 with 4-byte X entries and
+    # dense blocks take column pairs per lane
+    for k, shift in ((1, 0), (6, 0), (8, 0), (8, 1), (10, 0), (16, 1), (64, 0)):
         X = matgen.uniform_f64(4, i.n_block_cols * wp * k).astype(xt)
         n_y = i.n_block_rows * hp * k
         dX = ctx.upload(np.concatenate([X[:shift], X]))
