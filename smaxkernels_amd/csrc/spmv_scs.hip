@@ -190,7 +190,12 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_w64(
 // chunk over several workgroups; chunks in the optional execution `order`;
 // LEN and the tail waves (C <= 256 only; their arrays are a SELL-64 whatever
 // C is) as in scs_spmv_w64.
-template <int VT, bool UNPERM, bool LEN, bool REMAP>
+//   CS > 0: C fixed at compile time (layouts without tail waves): the chain
+//   then addresses its elements with constant offsets and runs the C = 64
+//   kernel's group size (4M-row band matrix, C = 32, FP64: sigma = 65536
+//   0.692 -> 0.660 ms, 0.640 with the occupancy cap; sigma = N 1.093 ->
+//   1.061 ms; BF16 values, sigma = 65536: 0.495 -> 0.477 ms).
+template <int VT, bool UNPERM, bool LEN, bool REMAP, int CS>
 __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
     int64_t C, int64_t n_chunks, int64_t n_rows, int64_t cpb, int64_t bpc,
     const int32_t *__restrict__ chunk_ptr, const int32_t *__restrict__ chunk_len,
@@ -204,7 +209,8 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
     const int32_t *__restrict__ t_col, const typename vt<VT>::val_t *__restrict__ t_val,
     int64_t t_blocks) {
     using T = vt<VT>;
-    const bool tails = LEN && t_slot != nullptr;
+    if constexpr (CS > 0) C = CS;
+    const bool tails = CS == 0 && LEN && t_slot != nullptr;
     int64_t slot, base, stride;
     int clen, len;
     const int32_t *cp;
@@ -248,7 +254,11 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmv_generic(
         cp = col;
         vp = val;
     }
-    auto acc = slot_chain_pipe<VT, kScsUnroll>(cp, vp, x, base, stride, len);
+    typename T::vec_t acc;
+    if constexpr (CS > 0)
+        acc = slot_chain_pipe<VT, kScsPipeUnroll<VT>>(cp, vp, x, base, CS, len);
+    else
+        acc = slot_chain_pipe<VT, kScsUnroll>(cp, vp, x, base, stride, len);
     if (LEN && len < clen) acc = T::fma_(T::widen(typename T::val_t(0)), x[0], acc);
     if (UNPERM) {
         const int32_t r = slot_row[slot];
@@ -595,12 +605,18 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
     const dim3 grid(static_cast<unsigned>(blocks));
     const bool rm = remap && cpb > 0;
     auto tv = static_cast<const typename T::val_t *>(t ? t->val : nullptr);
-#define SMAX_SCS_GENERIC(U, L, R)                                                        \
-    hipLaunchKernelGGL((scs_spmv_generic<VT, U, L, R>), grid, dim3(kScsThreads), 0,     \
+    // C = 32 without tail waves: the compile-time chunk width, and the
+    // schedule's occupancy cap
+    const bool c32 = C == 32 && !t;
+    const unsigned lds = c32 ? static_cast<unsigned>(lds_bytes) : 0u;
+#define SMAX_SCS_GENERIC_CS(U, L, R, CS)                                                 \
+    hipLaunchKernelGGL((scs_spmv_generic<VT, U, L, R, CS>), grid, dim3(kScsThreads), lds, \
                        ctx->stream, C, n_chunks, n_rows, cpb, bpc, chunk_ptr,           \
                        chunk_len, slot_len, order, col, v, slot_row, xv, yv,            \
                        t ? t->head : nullptr, t ? t->slot : nullptr, t ? t->ptr : nullptr, \
                        t ? t->col : nullptr, tv, tb)
+#define SMAX_SCS_GENERIC(U, L, R)                                                        \
+    if (c32) SMAX_SCS_GENERIC_CS(U, L, R, 32); else SMAX_SCS_GENERIC_CS(U, L, R, 0)
     if (unperm && slot_len) {
         if (rm) SMAX_SCS_GENERIC(true, true, true); else SMAX_SCS_GENERIC(true, true, false);
     } else if (unperm) {
@@ -611,6 +627,7 @@ int launch_scs_spmv(smax_hip_ctx ctx, int64_t C, int64_t n_chunks,
         if (rm) SMAX_SCS_GENERIC(false, false, true); else SMAX_SCS_GENERIC(false, false, false);
     }
 #undef SMAX_SCS_GENERIC
+#undef SMAX_SCS_GENERIC_CS
     SMAX_HIP_LAUNCHED(ctx, "scs_spmv_generic");
     return SMAX_HIP_SUCCESS;
 }
@@ -665,9 +682,11 @@ int build_schedule(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_rows_pa
     // Occupancy: with little padding every lane of a wave has gathers in
     // flight, and five waves per SIMD congest the path from L2 to the MALL;
     // reserving LDS for four workgroups per CU (four waves per SIMD) runs
-    // faster (0.70 -> 0.63 ms on the 4M-row Lomax matrix at sigma = 65536).
-    // Padded layouts (fill > 1.5) keep lanes idle and want every wave.
-    if (C == kWave && nnz > 0 && 2 * n_elements <= 3 * nnz)
+    // faster (0.70 -> 0.63 ms on the 4M-row Lomax matrix at sigma = 65536;
+    // C = 32: 0.660 -> 0.640 ms).  Padded layouts (fill > 1.5) keep lanes
+    // idle and want every wave (C = 32, sigma = 4096, fill 1.7: 0.745 ->
+    // 0.824 ms with the cap).
+    if ((C == kWave || C == 32) && nnz > 0 && 2 * n_elements <= 3 * nnz)
         out->lds_cap_bytes = kLdsPerCu / 4;
     const int64_t n = n_chunks;
     if (n <= 0) return SMAX_HIP_SUCCESS;

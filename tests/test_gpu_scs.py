@@ -199,7 +199,8 @@ def ragged(n, seed, max_len=40, empty=0.3):
 
 
 @pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
-@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 64), (64, 1 << 20), (32, 256)])
+@pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 64), (64, 1 << 20), (32, 256),
+                                      (32, 1 << 20)])
 @pytest.mark.parametrize("x0", [0.5, -0.0, np.inf, np.nan])
 def test_scs_apply_matches_full_chunk_chains(ctx, vt, C_, sigma, x0):
     """The handle-level SpMV stops each lane at its row's length; it must
