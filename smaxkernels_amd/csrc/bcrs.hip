@@ -193,10 +193,10 @@ __device__ __forceinline__ void load_block(const typename vt<VT>::val_t *__restr
 // the BH*Q chains give independent fmas.  Q = 16 bytes' worth (16-byte runs:
 // k a multiple of Q, X and Y 16-byte aligned) reads each lane's part of an X
 // row with one 16-byte load; Q = 2 with 4-byte entries takes column pairs
-// (no alignment needed).  Blocks go in groups of G: the G column indices, then all G
-// blocks' values and X entries are requested before any fma, so a row of
-// G blocks costs two memory round trips.  Per output the fma order is still
-// the reference's (block j, then jx).
+// (no alignment needed).  Blocks go in groups of G: the G column indices,
+// then all G blocks' values and X entries are requested before any fma, so a
+// row of G blocks costs two memory round trips.  Per output the fma order is
+// still the reference's (block j, then jx).
 template <int VT, int BH, int BW, int Q, bool DENSE, bool CM, int G>
 __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     int64_t n_block_rows, int h_pad, int w_pad, int col_major, int64_t k,

@@ -12,8 +12,8 @@ struct smax_hip_scs_sched {
     // SpMM and C != 64 SpMV: long chunks first, longest first, then the rest
     // in chunk order; nullptr = chunk order.
     int32_t *order_lf = nullptr;
-    // Dynamic LDS reserved by the C = 64 SpMV launch to cap its occupancy
-    // (0 = no cap).
+    // Dynamic LDS reserved by the C = 64 (and C = 32) SpMV launch to cap its
+    // occupancy (0 = no cap).
     int64_t lds_cap_bytes = 0;
     // Longest chunk (-1: unknown).  Layouts whose chunks all fit one short
     // group run the C = 64 SpMV's single-group chains.
