@@ -279,6 +279,25 @@ def test_scs_apply_short_chunks(ctx, vt, sigma, x0):
     s.destroy()
 
 
+@pytest.mark.parametrize("C_", [64, 32])
+def test_scs_apply_matrix_without_entries(ctx, C_):
+    """A matrix with no stored entries (every chunk of length 0) gives y = +0
+    through the handle, in both orders, whatever x holds."""
+    Z = Csr(100, 100, np.zeros(101, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    s = gpu_scs(ctx, Z, C_, 64)
+    x = np.full(100, np.nan)
+    dx = ctx.upload(x)
+    dy = ctx.upload(np.full(s.info.n_rows_padded, 7.0))
+    s.apply(capi.F64, dx, dy)
+    assert_bitwise(ctx.download(dy), np.zeros(s.info.n_rows_padded))
+    dy2 = ctx.upload(np.full(100, 7.0))
+    s.apply(capi.F64, dx, dy2, unpermuted=True)
+    assert_bitwise(ctx.download(dy2), np.zeros(100))
+    for d in (dx, dy, dy2):
+        d.free()
+    s.destroy()
+
+
 @pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 256), (64, 1 << 20), (32, 512), (8, 1)])
 def test_scs_spmv_plan_on_caller_arrays(ctx, C_, sigma):
     """The plan path (schedule only, caller-owned arrays) equals the raw SELL
