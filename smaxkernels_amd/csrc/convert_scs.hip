@@ -771,9 +771,11 @@ int build_scs_from_sell(smax_hip_ctx ctx, int64_t C, int64_t sigma, int64_t n_ro
                         int64_t val_bytes, const int32_t *perm, smax_hip_scs *out) {
     SMAX_HIP_REQUIRE(ctx, out, "smax_hip_scs_from_sell: null output");
     *out = nullptr;
-    SMAX_HIP_REQUIRE(ctx, C > 0 && sigma > 0 && n_chunks > 0 && n_rows_padded == n_chunks * C &&
-                              n_rows > 0 && n_rows <= n_rows_padded && n_cols > 0 &&
-                              n_elements >= 0 && nnz >= 0 && nnz <= n_elements,
+    // n_rows_padded == n_chunks * C, tested without forming the product
+    SMAX_HIP_REQUIRE(ctx, C > 0 && sigma > 0 && n_chunks > 0 && n_rows_padded % C == 0 &&
+                              n_rows_padded / C == n_chunks && n_rows > 0 &&
+                              n_rows <= n_rows_padded && n_cols > 0 && n_elements >= 0 &&
+                              nnz >= 0 && nnz <= n_elements,
                      "smax_hip_scs_from_sell: inconsistent sizes");
     SMAX_HIP_REQUIRE(ctx, val_bytes == 2 || val_bytes == 4 || val_bytes == 8,
                      "smax_hip_scs_from_sell: val_bytes must be 2, 4 or 8");

@@ -615,6 +615,19 @@ def test_scs_from_sell_refusals(ctx):
                        ("perm duplicate", {"perm": dup}), ("perm range", {"perm": oob})):
         st, h = build(**over)
         assert st == capi.ERROR_INVALID_VALUE and h is None, name
+    # sizes that do not fit together, including a chunk count whose product
+    # with C would overflow 64 bits
+    d = {k: ctx.upload(a[k]) for k in ("chunk_ptr", "chunk_lengths", "col", "val", "perm")}
+    for C_, npad, nch in ((i.C, i.n_rows_padded + 1, i.n_chunks), (1 << 40, i.n_rows_padded,
+                                                                   1 << 40)):
+        h = capi.vp()
+        st = lib.smax_hip_scs_from_sell(ctx.handle, C_, i.sigma, i.n_rows, npad, i.n_cols, nch,
+                                        i.n_elements, i.nnz, d["chunk_ptr"].ptr,
+                                        d["chunk_lengths"].ptr, d["col"].ptr, d["val"].ptr, 8,
+                                        d["perm"].ptr, capi.C.byref(h))
+        assert st == capi.ERROR_INVALID_VALUE and h.value is None, (C_, npad, nch)
+    for v in d.values():
+        v.free()
     st, h = build()
     assert st == capi.SUCCESS
     T = capi.Scs(ctx, h)
