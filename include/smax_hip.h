@@ -44,6 +44,13 @@ typedef enum {
 } smax_hip_status;
 
 /*
+ * Size limits.  Row, column and slot counts are int32; a product over k
+ * right-hand sides (SpMM, SDDMM) also needs k <= INT32_MAX and
+ * rows * k <= 2^40 (one thread per output in a grid of 2^32 workgroups).
+ * Larger sizes return SMAX_HIP_ERROR_OVERFLOW before anything is read.
+ */
+
+/*
  * Value types.  Replaces SMAX::FloatType {FLOAT32, FLOAT64}
  * (src/common.hpp:28) and adds BF16 value storage.
  *   F64 : f64 values, f64 x/y, f64 accumulation

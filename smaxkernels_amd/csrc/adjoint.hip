@@ -19,6 +19,7 @@
 // loads.  Other k take one group per stored element, with the row of each
 // element from a per-element row array built on first use.  Both kernels
 // evaluate the same tree, so dvals do not depend on which one runs.
+#include <algorithm>
 #include <utility>
 
 #include "common.hpp"
@@ -404,6 +405,9 @@ int smax_hip_scs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                      "smax_hip_scs_sddmm: the matrix was built from SELL arrays and has no CRS "
                      "positions; build it with smax_hip_scs_from_crs");
     const smax_hip_scs_info &in = A->info;
+    if (!extent_ok(in.n_rows_padded, k) || !extent_ok(std::max<int64_t>(in.n_elements, 1), k))
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW,
+                    "smax_hip_scs_sddmm: k = %lld exceeds the launch limits", (long long)k);
     if (in.n_elements == 0) return SMAX_HIP_SUCCESS;
     // the row kernel's 16-byte loads need 16-byte aligned G and X rows
     const int64_t run_bytes = sddmm_run(k) * (value_type == SMAX_HIP_F64 ? 8 : 4);

@@ -532,6 +532,9 @@ static int check_bcrs(smax_hip_ctx ctx, const char *fn, int value_type,
     SMAX_HIP_REQUIRE(ctx, b_h <= kBThreads && h_pad * w_pad <= kBTileVals,
                      "%s: blocks larger than %d values are not supported", fn,
                      kBTileVals);
+    if (nbr > INT32_MAX)
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW, "%s: %lld block rows exceed int32", fn,
+                    (long long)nbr);
     return SMAX_HIP_SUCCESS;
 }
 
@@ -585,6 +588,10 @@ int smax_hip_bcrs_spmm(smax_hip_ctx ctx, int value_type,
     if (rc) return rc;
     SMAX_HIP_REQUIRE(ctx, k > 0 && row_ptr && X && Y,
                      "smax_hip_bcrs_spmm: bad k or null pointer");
+    if (!extent_ok(n_block_rows * h_pad, k))
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW,
+                    "smax_hip_bcrs_spmm: %lld rows x %lld columns exceed the launch limits",
+                    (long long)(n_block_rows * h_pad), (long long)k);
     const int64_t threads = n_block_rows * b_h * k;
     const bool cm = block_col_major != 0;
     // register-blocked shapes: every (b_h, b_w) in {1, 2, 4} x {1, 2, 4}
@@ -969,6 +976,10 @@ int smax_hip_bcrs_sddmm(smax_hip_ctx ctx, int value_type, smax_hip_bcrs A, int64
                     value_type);
     SMAX_HIP_REQUIRE(ctx, A && k > 0 && G && X && dvals,
                      "smax_hip_bcrs_sddmm: bad arguments");
+    if (!extent_ok(A->info.n_block_rows * A->info.h_pad, k) ||
+        !extent_ok(A->info.n_block_cols * A->info.w_pad, k))
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW,
+                    "smax_hip_bcrs_sddmm: k = %lld exceeds the launch limits", (long long)k);
     if (A->info.n_blocks == 0) return SMAX_HIP_SUCCESS;
     int g = 1;
     while (g < k && g < 64) g <<= 1;

@@ -156,6 +156,14 @@ __device__ __forceinline__ void store_run(A *__restrict__ p, const A (&in)[Q]) {
     }
 }
 
+// Whether `rows` output rows of `k` columns fit the launch arithmetic: both
+// counts within int32, and one thread per (row, column) within a grid of
+// 2^32 workgroups of 256 (the products are formed only after the bounds).
+inline bool extent_ok(int64_t rows, int64_t k) {
+    return rows > 0 && k > 0 && rows <= INT32_MAX && k <= INT32_MAX &&
+           rows * k <= (static_cast<int64_t>(1) << 40);
+}
+
 // Whether every pointer is 16-byte aligned (the run helpers' precondition).
 inline bool aligned16(const void *a, const void *b) {
     return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;

@@ -691,6 +691,9 @@ int smax_hip_crs_spmm(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                     "smax_hip_crs_spmm: value type %d", value_type);
     SMAX_HIP_REQUIRE(ctx, n_rows > 0 && n_cols > 0 && nnz >= 0 && k > 0,
                      "smax_hip_crs_spmm: bad shape");
+    if (!extent_ok(n_rows, k) || n_cols > INT32_MAX)
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW, "%s: %lld rows x %lld columns exceed the "
+                    "launch limits", __func__, (long long)n_rows, (long long)k);
     SMAX_HIP_REQUIRE(ctx, row_ptr && X && Y && (nnz == 0 || (col && val)),
                      "smax_hip_crs_spmm: null pointer");
     if (row_major)
@@ -773,6 +776,9 @@ int smax_hip_crs_spmm_noalloc(smax_hip_ctx ctx, int value_type, int64_t n_rows,
                     "smax_hip_crs_spmm_noalloc: value type %d", value_type);
     SMAX_HIP_REQUIRE(ctx, n_rows > 0 && n_cols > 0 && nnz >= 0 && k > 0,
                      "smax_hip_crs_spmm_noalloc: bad shape");
+    if (!extent_ok(n_rows, k) || n_cols > INT32_MAX)
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW, "%s: %lld rows x %lld columns exceed the "
+                    "launch limits", __func__, (long long)n_rows, (long long)k);
     SMAX_HIP_REQUIRE(ctx, row_ptr && X && Y && (nnz == 0 || (col && val)),
                      "smax_hip_crs_spmm_noalloc: null pointer");
     SMAX_HIP_REQUIRE(ctx, (ldx >= n_cols || ldx >= n_rows) && ldy >= n_rows,

@@ -945,7 +945,7 @@ static int check_scs_args(smax_hip_ctx ctx, const char *fn, int value_type,
         return fail(ctx, SMAX_HIP_ERROR_UNSUPPORTED, "%s: value type %d", fn,
                     value_type);
     SMAX_HIP_REQUIRE(ctx, C > 0 && n_chunks > 0, "%s: bad C/n_chunks", fn);
-    SMAX_HIP_REQUIRE(ctx, n_chunks * C <= INT32_MAX, "%s: too many slots", fn);
+    SMAX_HIP_REQUIRE(ctx, C <= INT32_MAX && n_chunks <= INT32_MAX / C, "%s: too many slots", fn);
     SMAX_HIP_REQUIRE(ctx, chunk_ptr && chunk_len && col && val && x && y,
                      "%s: null pointer", fn);
     return SMAX_HIP_SUCCESS;
@@ -1002,8 +1002,9 @@ int smax_hip_scs_plan_create(smax_hip_ctx ctx, int64_t C, int64_t sigma,
     SMAX_HIP_REQUIRE(ctx, out, "smax_hip_scs_plan_create: null output");
     *out = nullptr;
     SMAX_HIP_REQUIRE(ctx, C > 0 && sigma > 0 && n_chunks > 0 && n_elements >= 0 &&
-                              nnz >= 0 && nnz <= n_elements &&
-                              n_rows_padded == n_chunks * C && chunk_lengths,
+                              nnz >= 0 && nnz <= n_elements && C <= INT32_MAX &&
+                              n_chunks <= INT32_MAX / C && n_rows_padded == n_chunks * C &&
+                              chunk_lengths,
                      "smax_hip_scs_plan_create: bad arguments");
     auto *plan = new smax_hip_scs_plan_s();
     plan->C = C;
@@ -1102,6 +1103,9 @@ int smax_hip_scs_spmm(smax_hip_ctx ctx, int value_type, int64_t C,
     SMAX_HIP_REQUIRE(ctx, slot_row && k > 0 && n_rows > 0 &&
                               n_rows <= n_chunks * C,
                      "smax_hip_scs_spmm: bad slot_row / n_rows / k");
+    if (!extent_ok(n_chunks * C, k))
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW,
+                    "smax_hip_scs_spmm: k = %lld exceeds the launch limits", (long long)k);
     SMAX_HIP_DISPATCH_VT(value_type,
                          return launch_scs_spmm<VT>(ctx, C, n_chunks, n_rows, k, chunk_ptr,
                                                     chunk_lengths, nullptr, nullptr, col,
@@ -1122,6 +1126,9 @@ int smax_hip_scs_apply_spmm(smax_hip_ctx ctx, int value_type, smax_hip_scs A,
                             X, Y);
     if (rc) return rc;
     SMAX_HIP_REQUIRE(ctx, k > 0, "smax_hip_scs_apply_spmm: k must be positive");
+    if (!extent_ok(in.n_rows_padded, k))
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW,
+                    "smax_hip_scs_apply_spmm: k = %lld exceeds the launch limits", (long long)k);
     if (vt_val_bytes(value_type) != in.val_bytes)
         return fail(ctx, SMAX_HIP_ERROR_INVALID_VALUE,
                     "smax_hip_scs_apply_spmm: value type %d does not match %lld-byte "
