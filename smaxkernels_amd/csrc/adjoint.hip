@@ -357,6 +357,10 @@ int smax_hip_scs_transpose_from_crs(smax_hip_ctx ctx, int64_t n_rows,
                               nnz <= INT32_MAX && row_ptr &&
                               (nnz == 0 || (col && val)),
                      "smax_hip_scs_transpose_from_crs: bad input");
+    if (n_rows > INT32_MAX || n_cols > INT32_MAX)
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW,
+                    "smax_hip_scs_transpose_from_crs: %lld x %lld exceeds int32",
+                    (long long)n_rows, (long long)n_cols);
     const int64_t m = nnz > 0 ? nnz : 1;
     uint32_t *keys = nullptr;
     int32_t *vals = nullptr, *t_row_ptr = nullptr, *t_col = nullptr;

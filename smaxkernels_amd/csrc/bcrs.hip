@@ -652,6 +652,10 @@ int smax_hip_bcrs_from_crs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols,
     SMAX_HIP_REQUIRE(ctx, n_rows > 0 && n_cols > 0 && nnz >= 0 && row_ptr &&
                               (nnz == 0 || (col && val)),
                      "smax_hip_bcrs_from_crs: bad input");
+    if (n_rows > INT32_MAX || n_cols > INT32_MAX || nnz > INT32_MAX)
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW,
+                    "smax_hip_bcrs_from_crs: %lld x %lld with %lld entries exceeds int32",
+                    (long long)n_rows, (long long)n_cols, (long long)nnz);
     SMAX_HIP_REQUIRE(ctx, val_bytes == 2 || val_bytes == 4 || val_bytes == 8,
                      "smax_hip_bcrs_from_crs: val_bytes must be 2, 4 or 8");
     // convert_formats.hpp:142-157

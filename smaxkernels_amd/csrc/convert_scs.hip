@@ -652,10 +652,10 @@ int build_scs(smax_hip_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
                      "scs conversion: val_bytes must be 2, 4 or 8");
     SMAX_HIP_REQUIRE(ctx, row_ptr && (nnz == 0 || (col && val)),
                      "scs conversion: null input array");
-    if (n_rows > INT32_MAX || C > INT32_MAX || nnz > INT32_MAX)
+    if (n_rows > INT32_MAX || n_cols > INT32_MAX || C > INT32_MAX || nnz > INT32_MAX)
         return fail(ctx, SMAX_HIP_ERROR_OVERFLOW,
-                    "scs conversion: %lld rows, C = %lld or %lld entries exceed int32",
-                    (long long)n_rows, (long long)C, (long long)nnz);
+                    "scs conversion: %lld x %lld, C = %lld or %lld entries exceed int32",
+                    (long long)n_rows, (long long)n_cols, (long long)C, (long long)nnz);
     const int64_t n_chunks = ceil_div(n_rows, C);
     const int64_t n = n_chunks * C;
     if (n > INT32_MAX)

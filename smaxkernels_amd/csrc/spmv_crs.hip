@@ -648,6 +648,9 @@ int smax_hip_crs_spmv(smax_hip_ctx ctx, int value_type, int64_t n_rows,
     SMAX_HIP_REQUIRE(ctx, n_rows > 0 && n_cols > 0 && nnz >= 0 &&
                               nnz <= INT32_MAX,
                      "smax_hip_crs_spmv: bad shape");
+    if (n_rows > INT32_MAX || n_cols > INT32_MAX)
+        return fail(ctx, SMAX_HIP_ERROR_OVERFLOW, "smax_hip_crs_spmv: %lld x %lld exceeds int32",
+                    (long long)n_rows, (long long)n_cols);
     SMAX_HIP_REQUIRE(ctx, row_ptr && x && y && (nnz == 0 || (col && val)),
                      "smax_hip_crs_spmv: null pointer");
     if (plan)

@@ -53,6 +53,21 @@ def test_products_refuse_sizes_beyond_the_launch_limits(ctx):
     st = lib.smax_hip_bcrs_sddmm(ctx.handle, capi.F64, b.handle, BIG, X.ptr, X.ptr, Y.ptr)
     assert st == capi.ERROR_OVERFLOW
     b.destroy()
+    # row and column counts beyond int32: SpMV and the three conversions
+    huge = (1 << 31) + 4
+    st = lib.smax_hip_crs_spmv(ctx.handle, capi.F64, huge, A.n_cols, A.nnz, rp.ptr, col.ptr,
+                               v.ptr, X.ptr, Y.ptr, None)
+    assert st == capi.ERROR_OVERFLOW
+    h = capi.vp()
+    st = lib.smax_hip_scs_from_crs(ctx.handle, A.n_rows, huge, A.nnz, rp.ptr, col.ptr, v.ptr, 8,
+                                   64, 64, capi.C.byref(h))
+    assert st == capi.ERROR_OVERFLOW and h.value is None
+    st = lib.smax_hip_scs_transpose_from_crs(ctx.handle, A.n_rows, huge, A.nnz, rp.ptr,
+                                             col.ptr, v.ptr, 8, 64, 64, capi.C.byref(h))
+    assert st == capi.ERROR_OVERFLOW and h.value is None
+    st = lib.smax_hip_bcrs_from_crs(ctx.handle, huge, huge, B.nnz, brp.ptr, bcol.ptr, bv.ptr, 8,
+                                    4, 4, 4, 4, 0, capi.C.byref(h))
+    assert st == capi.ERROR_OVERFLOW and h.value is None
     ctx.sync()
     assert (ctx.download(Y) == 5.0).all()
     for d in (rp, col, v, X, Y, brp, bcol, bv):
