@@ -15,6 +15,7 @@
 #include "scan.hpp"
 
 #include <type_traits>
+#include <utility>
 
 struct smax_hip_bcrs_s {
     smax_hip_bcrs_info info;
@@ -249,6 +250,85 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_reg(
     for (int ix = 0; ix < BH; ++ix) store_run<A, Q>(Y + (r * h_pad + ix) * k + v, acc[ix]);
 }
 
+// Lane N of each 16-lane DPP row, handed to every lane of that row
+// (row_newbcast).
+template <int N>
+__device__ __forceinline__ float row_bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + N, 0xf, 0xf, false));
+}
+
+template <int... N>
+__device__ __forceinline__ void row_bcast16(float mine, float (&out)[16],
+                                            std::integer_sequence<int, N...>) {
+    ((out[N] = row_bcast<N>(mine)), ...);
+}
+
+// bcrs_spmm_reg for dense 4x4 blocks and k = 64 four-byte columns, where a
+// block row's 16 lanes (four columns each) are exactly one DPP row.  There,
+// every lane loading the whole 16-value block makes a third of the bytes the
+// texture data path returns, and that path is what the kernel waits on
+// (TD busy ~98 % of the cycles on the 96^3 block matrix).  Here each lane
+// loads one value of the block and 16 row broadcasts give every lane all of
+// them; the next group's column indices are requested while this group's X
+// rows are in flight.  Per output the fma order is bcrs_spmm_reg's (block j,
+// then jx), so Y is bit-identical.
+template <int VT, bool CM, int G>
+__global__ __launch_bounds__(kBThreads) void bcrs_spmm_bcast(
+    int64_t n_block_rows, int64_t k, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ row_ptr, const typename vt<VT>::val_t *__restrict__ val,
+    const float *__restrict__ X, float *__restrict__ Y) {
+    using T = vt<VT>;
+    constexpr int BH = 4, BW = 4, Q = 4, kRow = 16;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
+    const int64_t r = t / kRow;  // whole DPP rows leave together
+    const int l = static_cast<int>(threadIdx.x) % kRow;
+    const int64_t v = static_cast<int64_t>(l) * Q;
+    if (r >= n_block_rows) return;
+    float acc[BH][Q];
+#pragma unroll
+    for (int ix = 0; ix < BH; ++ix)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[ix][q] = 0;
+    const int64_t jb = row_ptr[r], je = row_ptr[r + 1];
+    int32_t cn[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) cn[g] = jb + g < je ? col[jb + g] : 0;
+    for (int64_t j = jb; j < je; j += G) {
+        int32_t c[G];
+        float mine[G];
+        float xv[G][BW][Q];
+#pragma unroll
+        for (int g = 0; g < G; ++g) c[g] = cn[g];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (j + g < je) {
+                const float *xr = X + static_cast<int64_t>(c[g]) * BW * k + v;
+#pragma unroll
+                for (int jx = 0; jx < BW; ++jx) load_run<float, Q>(xr + jx * k, xv[g][jx]);
+                mine[g] = T::widen(val[(j + g) * (BH * BW) + l]);
+            }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (j + G + g < je) cn[g] = col[j + G + g];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (j + g < je) {
+                float w[BH * BW];
+                row_bcast16(mine[g], w, std::make_integer_sequence<int, BH * BW>{});
+#pragma unroll
+                for (int ix = 0; ix < BH; ++ix)
+#pragma unroll
+                    for (int jx = 0; jx < BW; ++jx)
+#pragma unroll
+                        for (int q = 0; q < Q; ++q)
+                            acc[ix][q] = T::fma_(w[CM ? jx * BH + ix : ix * BW + jx],
+                                                 xv[g][jx][q], acc[ix][q]);
+            }
+    }
+#pragma unroll
+    for (int ix = 0; ix < BH; ++ix) store_run<float, Q>(Y + (r * BH + ix) * k + v, acc[ix]);
+}
+
 template <int VT, int BH, int BW>
 static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t h_pad,
                                  int64_t w_pad, bool cm, int64_t k, const int32_t *col,
@@ -274,6 +354,21 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
     const bool pairs = !runs && dense && sizeof(A) == 4 && k % 2 == 0 && k >= 8;
     const int64_t lanes = runs ? k / Q : pairs ? k / 2 : k;  // per block row
     const dim3 g = grid_for(n_block_rows * lanes, kBThreads);
+    // 4x4 dense blocks at k = 64 (16 lanes per block row): one value per lane
+    // and DPP row broadcasts, in groups of 4 blocks (96^3 block matrix, BF16
+    // values: 0.646 -> 0.617 ms forward and through the transpose; groups of
+    // 2: 0.621 ms)
+    if constexpr (BH == 4 && BW == 4 && sizeof(A) == 4) {
+        if (runs && dense && lanes == 16) {
+            if (cm)
+                hipLaunchKernelGGL((bcrs_spmm_bcast<VT, true, 4>), g, dim3(kBThreads), 0,
+                                   ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
+            else
+                hipLaunchKernelGGL((bcrs_spmm_bcast<VT, false, 4>), g, dim3(kBThreads), 0,
+                                   ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
+            return;
+        }
+    }
     // block groups: 8 when a single thread holds the whole block row (k = 1),
     // else 2 (measured on the 96^3 block matrix: k = 1 0.108 -> 0.058 ms,
     // k = 8 0.173 -> 0.160 ms, k = 64 0.663 -> 0.649 ms in BF16).  Padded

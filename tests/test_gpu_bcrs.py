@@ -116,6 +116,36 @@ def test_block7_conversion_spmv_spmm(ctx, vt, bh, bw, hp, wp, cm):
     b.destroy()
 
 
+@pytest.mark.parametrize("vt", [capi.F32, capi.BF16])
+@pytest.mark.parametrize("cm", [True, False])
+def test_bcrs_spmm_k64_ragged_block_rows(ctx, vt, cm):
+    """Dense 4x4 blocks at k = 64 (one value per lane, broadcast across the
+    block row's 16 lanes): empty block rows, rows of 1-9 blocks (every
+    remainder of the 4-block groups) and rows of 40 blocks, with a block row
+    count that leaves the last workgroup partly empty; bit for bit against
+    the oracle."""
+    rng = np.random.default_rng(64)
+    nbr, nbc, k = 203, 300, 64
+    lens = rng.integers(1, 10, nbr)
+    lens[::5] = 0
+    lens[3::17] = 40
+    row_ptr = np.zeros(nbr + 1, np.int32)
+    row_ptr[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(nbc, n, replace=False)) for n in lens]
+                         ).astype(np.int32)
+    val = vals_for(vt, rng.uniform(-1.0, 1.0, int(row_ptr[-1]) * 16))
+    X = rng.uniform(-1.0, 1.0, nbc * 4 * k).astype(np.float32)
+    d = [ctx.upload(a) for a in (col, row_ptr, val, X)]
+    dY = ctx.upload(np.full(nbr * 4 * k, 7.0, np.float32))
+    ctx.bcrs_spmm(vt, nbr, 4, 4, 4, 4, d[0], d[1], d[2], cm, k, d[3], dY)
+    want = oracle.bcrs_spmm(nbr, 4, 4, 4, 4, col, row_ptr, val, cm, k, X)
+    got = ctx.download(dY)
+    assert_bitwise(got, want, "bcrs spmm k=64")
+    assert (got.reshape(nbr, -1)[lens == 0] == 0).all()
+    for a in (*d, dY):
+        a.free()
+
+
 def test_bcrs_scalar_equivalence(ctx):
     """BCRS SpMV of the 7-point block matrix equals the CRS SpMV up to
     summation order (rows of the block matrix padded by zeros)."""
