@@ -266,11 +266,11 @@ __device__ __forceinline__ void row_bcast16(float mine, float (&out)[16],
 // bcrs_spmm_reg for dense 4x4 blocks and k = 64 four-byte columns, where a
 // block row's 16 lanes (four columns each) are exactly one DPP row.  There,
 // every lane loading the whole 16-value block makes a third of the bytes the
-// texture data path returns, and that path is what the kernel waits on
-// (TD busy ~98 % of the cycles on the 96^3 block matrix).  Here each lane
-// loads one value of the block and 16 row broadcasts give every lane all of
-// them; the next group's column indices are requested while this group's X
-// rows are in flight.  Per output the fma order is bcrs_spmm_reg's (block j,
+// vector memory path (texture address unit busy ~88 % of the kernel's cycles
+// on the 96^3 block matrix) hands back.  Here each lane loads one value of
+// the block and 16 row broadcasts give every lane all of them; the next
+// group's column indices are requested while this group's X rows are in
+// flight.  Per output the fma order is bcrs_spmm_reg's (block j,
 // then jx), so Y is bit-identical.
 template <int VT, bool CM, int G>
 __global__ __launch_bounds__(kBThreads) void bcrs_spmm_bcast(
