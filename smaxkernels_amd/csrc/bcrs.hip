@@ -329,6 +329,98 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_bcast(
     for (int ix = 0; ix < BH; ++ix) store_run<float, Q>(Y + (r * BH + ix) * k + v, acc[ix]);
 }
 
+// Dword v of lane S of each DPP quad, handed to the quad's four lanes
+// (quad_perm [S,S,S,S]).
+template <int S>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(v), S * 0x55, 0xf, 0xf, false));
+}
+
+// bcrs_spmm_reg's column pairs for dense 4x4 blocks at k = 8 four-byte
+// columns, where a block row's 4 lanes are one DPP quad: each lane loads a
+// quarter of the block (8 bytes of BF16 values, 16 of FP32) and quad
+// broadcasts give every lane the whole block, instead of every lane loading
+// all of it.  Blocks go in groups of 4 (the quarters take few registers),
+// with the next group's column indices requested while this group's X rows
+// are in flight.  Per output the fma order is bcrs_spmm_reg's (block j, then
+// jx), so Y is bit-identical.
+template <int VT, bool CM, int G>
+__global__ __launch_bounds__(kBThreads) void bcrs_spmm_quad(
+    int64_t n_block_rows, int64_t k, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ row_ptr, const typename vt<VT>::val_t *__restrict__ val,
+    const float *__restrict__ X, float *__restrict__ Y) {
+    using T = vt<VT>;
+    using V = typename T::val_t;
+    constexpr int BH = 4, BW = 4, Q = 2, kQuad = 4;
+    constexpr int kWords = BH * BW * static_cast<int>(sizeof(V)) / 4;  // dwords per block
+    constexpr int kMine = kWords / kQuad;                              // loaded per lane
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
+    const int64_t r = t / kQuad;  // whole quads leave together
+    const int l = static_cast<int>(threadIdx.x) % kQuad;
+    const int64_t v = static_cast<int64_t>(l) * Q;
+    if (r >= n_block_rows) return;
+    float acc[BH][Q];
+#pragma unroll
+    for (int ix = 0; ix < BH; ++ix)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[ix][q] = 0;
+    const int64_t jb = row_ptr[r], je = row_ptr[r + 1];
+    int32_t cn[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) cn[g] = jb + g < je ? col[jb + g] : 0;
+    const uint32_t *vw = reinterpret_cast<const uint32_t *>(val);
+    for (int64_t j = jb; j < je; j += G) {
+        int32_t c[G];
+        uint32_t mine[G][kMine];
+        float xv[G][BW][Q];
+#pragma unroll
+        for (int g = 0; g < G; ++g) c[g] = cn[g];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (j + g < je) {
+                const float *xr = X + static_cast<int64_t>(c[g]) * BW * k + v;
+#pragma unroll
+                for (int jx = 0; jx < BW; ++jx) load_run<float, Q>(xr + jx * k, xv[g][jx]);
+                load_run<uint32_t, kMine>(vw + (j + g) * kWords + l * kMine, mine[g]);
+            }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (j + G + g < je) cn[g] = col[j + G + g];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (j + g < je) {
+                uint32_t words[kWords];
+#pragma unroll
+                for (int d = 0; d < kMine; ++d) {
+                    words[0 * kMine + d] = quad_bcast<0>(mine[g][d]);
+                    words[1 * kMine + d] = quad_bcast<1>(mine[g][d]);
+                    words[2 * kMine + d] = quad_bcast<2>(mine[g][d]);
+                    words[3 * kMine + d] = quad_bcast<3>(mine[g][d]);
+                }
+                float w[BH * BW];
+#pragma unroll
+                for (int i = 0; i < BH * BW; ++i) {
+                    if constexpr (sizeof(V) == 2)  // BF16: two values per dword, low first
+                        w[i] = __uint_as_float((i & 1) ? (words[i >> 1] & 0xffff0000u)
+                                                       : (words[i >> 1] << 16));
+                    else
+                        w[i] = __uint_as_float(words[i]);
+                }
+#pragma unroll
+                for (int ix = 0; ix < BH; ++ix)
+#pragma unroll
+                    for (int jx = 0; jx < BW; ++jx)
+#pragma unroll
+                        for (int q = 0; q < Q; ++q)
+                            acc[ix][q] = T::fma_(w[CM ? jx * BH + ix : ix * BW + jx],
+                                                 xv[g][jx][q], acc[ix][q]);
+            }
+    }
+#pragma unroll
+    for (int ix = 0; ix < BH; ++ix) store_run<float, Q>(Y + (r * BH + ix) * k + v, acc[ix]);
+}
+
 template <int VT, int BH, int BW>
 static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t h_pad,
                                  int64_t w_pad, bool cm, int64_t k, const int32_t *col,
@@ -365,6 +457,19 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
                                    ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
             else
                 hipLaunchKernelGGL((bcrs_spmm_bcast<VT, false, 4>), g, dim3(kBThreads), 0,
+                                   ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
+            return;
+        }
+        // k = 8 (4 lanes per block row): a quarter of each block per lane and
+        // DPP quad broadcasts, in groups of 4 blocks (BF16 values 0.152 ->
+        // 0.141 ms forward and through the transpose, FP32 0.183 -> 0.163 ms;
+        // groups of 2: 0.150 / 0.171 ms)
+        if (pairs && lanes == 4) {
+            if (cm)
+                hipLaunchKernelGGL((bcrs_spmm_quad<VT, true, 4>), g, dim3(kBThreads), 0,
+                                   ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
+            else
+                hipLaunchKernelGGL((bcrs_spmm_quad<VT, false, 4>), g, dim3(kBThreads), 0,
                                    ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
             return;
         }

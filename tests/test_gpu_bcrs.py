@@ -118,14 +118,16 @@ def test_block7_conversion_spmv_spmm(ctx, vt, bh, bw, hp, wp, cm):
 
 @pytest.mark.parametrize("vt", [capi.F32, capi.BF16])
 @pytest.mark.parametrize("cm", [True, False])
-def test_bcrs_spmm_k64_ragged_block_rows(ctx, vt, cm):
+@pytest.mark.parametrize("k", [8, 64])
+def test_bcrs_spmm_ragged_block_rows(ctx, vt, cm, k):
     """Dense 4x4 blocks at k = 64 (one value per lane, broadcast across the
-    block row's 16 lanes): empty block rows, rows of 1-9 blocks (every
+    block row's 16 lanes) and k = 8 (a quarter block per lane, broadcast
+    across the row's 4 lanes): empty block rows, rows of 1-9 blocks (every
     remainder of the 4-block groups) and rows of 40 blocks, with a block row
     count that leaves the last workgroup partly empty; bit for bit against
     the oracle."""
     rng = np.random.default_rng(64)
-    nbr, nbc, k = 203, 300, 64
+    nbr, nbc = 203, 300
     lens = rng.integers(1, 10, nbr)
     lens[::5] = 0
     lens[3::17] = 40
@@ -140,7 +142,7 @@ def test_bcrs_spmm_k64_ragged_block_rows(ctx, vt, cm):
     ctx.bcrs_spmm(vt, nbr, 4, 4, 4, 4, d[0], d[1], d[2], cm, k, d[3], dY)
     want = oracle.bcrs_spmm(nbr, 4, 4, 4, 4, col, row_ptr, val, cm, k, X)
     got = ctx.download(dY)
-    assert_bitwise(got, want, "bcrs spmm k=64")
+    assert_bitwise(got, want, f"bcrs spmm k={k}")
     assert (got.reshape(nbr, -1)[lens == 0] == 0).all()
     for a in (*d, dY):
         a.free()
