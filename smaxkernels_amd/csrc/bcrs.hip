@@ -337,22 +337,23 @@ __device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
         __builtin_amdgcn_mov_dpp(static_cast<int>(v), S * 0x55, 0xf, 0xf, false));
 }
 
-// bcrs_spmm_reg's column pairs for dense 4x4 blocks at k = 8 four-byte
-// columns, where a block row's 4 lanes are one DPP quad: each lane loads a
+// bcrs_spmm_reg for dense 4x4 blocks with four-byte columns and 4 lanes per
+// block row (k = 8 in column pairs, Q = 2, or k = 16 in 16-byte runs, Q =
+// 4), so that a block row's lanes are one DPP quad: each lane loads a
 // quarter of the block (8 bytes of BF16 values, 16 of FP32) and quad
 // broadcasts give every lane the whole block, instead of every lane loading
 // all of it.  Blocks go in groups of 4 (the quarters take few registers),
 // with the next group's column indices requested while this group's X rows
 // are in flight.  Per output the fma order is bcrs_spmm_reg's (block j, then
 // jx), so Y is bit-identical.
-template <int VT, bool CM, int G>
+template <int VT, int Q, bool CM, int G>
 __global__ __launch_bounds__(kBThreads) void bcrs_spmm_quad(
     int64_t n_block_rows, int64_t k, const int32_t *__restrict__ col,
     const int32_t *__restrict__ row_ptr, const typename vt<VT>::val_t *__restrict__ val,
     const float *__restrict__ X, float *__restrict__ Y) {
     using T = vt<VT>;
     using V = typename T::val_t;
-    constexpr int BH = 4, BW = 4, Q = 2, kQuad = 4;
+    constexpr int BH = 4, BW = 4, kQuad = 4;
     constexpr int kWords = BH * BW * static_cast<int>(sizeof(V)) / 4;  // dwords per block
     constexpr int kMine = kWords / kQuad;                              // loaded per lane
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
@@ -460,19 +461,26 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
                                    ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
             return;
         }
-        // k = 8 (4 lanes per block row): a quarter of each block per lane and
-        // DPP quad broadcasts, in groups of 4 blocks (BF16 values 0.152 ->
-        // 0.141 ms forward and through the transpose, FP32 0.183 -> 0.163 ms;
-        // groups of 2: 0.150 / 0.171 ms)
+        // 4 lanes per block row (k = 8 in pairs, k = 16 in runs): a quarter of
+        // each block per lane and DPP quad broadcasts, in groups of 4 blocks
+        // (k = 8: BF16 values 0.152 -> 0.139 ms forward and through the
+        // transpose, FP32 0.183 -> 0.160 ms; groups of 2: 0.150 / 0.171 ms)
+#define SMAX_BCRS_QUAD(QQ)                                                                 \
+    if (cm)                                                                                \
+        hipLaunchKernelGGL((bcrs_spmm_quad<VT, QQ, true, 4>), g, dim3(kBThreads), 0,       \
+                           ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);         \
+    else                                                                                   \
+        hipLaunchKernelGGL((bcrs_spmm_quad<VT, QQ, false, 4>), g, dim3(kBThreads), 0,      \
+                           ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv)
         if (pairs && lanes == 4) {
-            if (cm)
-                hipLaunchKernelGGL((bcrs_spmm_quad<VT, true, 4>), g, dim3(kBThreads), 0,
-                                   ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
-            else
-                hipLaunchKernelGGL((bcrs_spmm_quad<VT, false, 4>), g, dim3(kBThreads), 0,
-                                   ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);
+            SMAX_BCRS_QUAD(2);
             return;
         }
+        if (runs && dense && lanes == 4) {
+            SMAX_BCRS_QUAD(4);
+            return;
+        }
+#undef SMAX_BCRS_QUAD
     }
     // block groups: 8 when a single thread holds the whole block row (k = 1),
     // else 2 (measured on the 96^3 block matrix: k = 1 0.108 -> 0.058 ms,
