@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "smax_hip.h"
@@ -115,6 +116,31 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks) {
     const int64_t xcd = bid % kXcd, local = bid / kXcd;
     const int64_t q = nblocks / kXcd, r = nblocks % kXcd;
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+// Lane U of each aligned group of 8 lanes, handed to the group (ds_swizzle
+// in bit mode: and-mask 0x18 keeps the group, or-mask U picks the lane).
+// Used where 8 lanes share one row or slot and would otherwise each load
+// the same column index and value.
+template <int U>
+__device__ __forceinline__ int grp8_bcast(int v) {
+    return __builtin_amdgcn_ds_swizzle(v, 0x18 | (U << 5));
+}
+template <int... U>
+__device__ __forceinline__ void grp8_bcast8(int c, int w, int (&cs)[8], int (&ws)[8],
+                                            std::integer_sequence<int, U...>) {
+    ((cs[U] = grp8_bcast<U>(c)), ...);
+    ((ws[U] = grp8_bcast<U>(w)), ...);
+}
+template <typename V>
+__device__ __forceinline__ int val_bits(V v) {
+    if constexpr (sizeof(V) == 4) return __float_as_int(v);
+    else return static_cast<int>(v);
+}
+template <typename V>
+__device__ __forceinline__ V bits_val(int b) {
+    if constexpr (sizeof(V) == 4) return __int_as_float(b);
+    else return static_cast<V>(b);
 }
 
 // Q consecutive entries at p.  When Q * sizeof(A) is a multiple of 16 the

@@ -239,7 +239,39 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_rowvec(
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = 0;
     int64_t j = s;
-    if (e - s >= U) {
+    if constexpr (GL == U && sizeof(V) <= 4) {
+        // 8 lanes per row (k = 32 with 4-byte X entries): lane l loads entry
+        // l of each group of 8 and a swizzle within the 8 lanes hands every
+        // lane all of them, instead of all 8 loading each; the next group's
+        // are requested before this group's X rows
+        if (e - s >= U) {
+            int cm = col[j + l];
+            int wm = val_bits<V>(val[j + l]);
+            for (;;) {
+                const bool more = j + 2 * U <= e;
+                int cn = 0, wn = 0;
+                if (more) {
+                    cn = col[j + U + l];
+                    wn = val_bits<V>(val[j + U + l]);
+                }
+                int cs[U], ws[U];
+                grp8_bcast8(cm, wm, cs, ws, std::make_integer_sequence<int, U>{});
+                A xv[U][Q];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    load_run<A, Q>(X + static_cast<int64_t>(cs[u]) * ldx + l * Q, xv[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+                        acc[q] = T::fma_(T::widen(bits_val<V>(ws[u])), xv[u][q], acc[q]);
+                j += U;
+                if (!more) break;
+                cm = cn;
+                wm = wn;
+            }
+        }
+    } else if (e - s >= U) {
         int32_t c0[U];
         V w0[U];
 #pragma unroll

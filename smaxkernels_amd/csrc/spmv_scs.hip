@@ -350,29 +350,6 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmm_kernel(
     Y[static_cast<int64_t>(r) * k + v] = acc;
 }
 
-// Lane U of each group of 8 lanes, handed to the group (ds_swizzle in bit
-// mode: and-mask 0x18 keeps the group, or-mask U picks the lane).
-template <int U>
-__device__ __forceinline__ int grp8_bcast(int v) {
-    return __builtin_amdgcn_ds_swizzle(v, 0x18 | (U << 5));
-}
-template <int... U>
-__device__ __forceinline__ void grp8_bcast8(int c, int w, int (&cs)[8], int (&ws)[8],
-                                            std::integer_sequence<int, U...>) {
-    ((cs[U] = grp8_bcast<U>(c)), ...);
-    ((ws[U] = grp8_bcast<U>(w)), ...);
-}
-template <typename V>
-__device__ __forceinline__ int val_bits(V v) {
-    if constexpr (sizeof(V) == 4) return __float_as_int(v);
-    else return static_cast<int>(v);
-}
-template <typename V>
-__device__ __forceinline__ V bits_val(int b) {
-    if constexpr (sizeof(V) == 4) return __int_as_float(b);
-    else return static_cast<V>(b);
-}
-
 // As scs_spmm_kernel for k = K a power of two with 16-byte aligned X and Y:
 // a lane owns Q = 16 / sizeof(vec) consecutive columns v (K / Q lanes per
 // slot), reads its part of each X row with one 16-byte load and keeps Q fma
