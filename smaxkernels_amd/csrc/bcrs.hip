@@ -449,7 +449,7 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
     const dim3 g = grid_for(n_block_rows * lanes, kBThreads);
     // 4x4 dense blocks at k = 64 (16 lanes per block row): one value per lane
     // and DPP row broadcasts, in groups of 4 blocks (96^3 block matrix, BF16
-    // values: 0.646 -> 0.617 ms forward and through the transpose; groups of
+    // values: 0.646 -> 0.615 ms forward and through the transpose; groups of
     // 2: 0.621 ms)
     if constexpr (BH == 4 && BW == 4 && sizeof(A) == 4) {
         if (runs && dense && lanes == 16) {
@@ -464,7 +464,8 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
         // 4 lanes per block row (k = 8 in pairs, k = 16 in runs): a quarter of
         // each block per lane and DPP quad broadcasts, in groups of 4 blocks
         // (k = 8: BF16 values 0.152 -> 0.139 ms forward and through the
-        // transpose, FP32 0.183 -> 0.160 ms; groups of 2: 0.150 / 0.171 ms)
+        // transpose, FP32 0.183 -> 0.160 ms; groups of 2: 0.150 / 0.171 ms;
+        // k = 16, FP32 values: 0.240 -> 0.223 ms)
 #define SMAX_BCRS_QUAD(QQ)                                                                 \
     if (cm)                                                                                \
         hipLaunchKernelGGL((bcrs_spmm_quad<VT, QQ, true, 4>), g, dim3(kBThreads), 0,       \
