@@ -132,6 +132,18 @@ __device__ __forceinline__ void grp8_bcast8(int c, int w, int (&cs)[8], int (&ws
     ((cs[U] = grp8_bcast<U>(c)), ...);
     ((ws[U] = grp8_bcast<U>(w)), ...);
 }
+// The same for 16-lane groups (a DPP row): lane U of the row, U < 8, handed
+// to all 16 lanes (row_newbcast).
+template <int U>
+__device__ __forceinline__ int row16_bcast(int v) {
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + U, 0xf, 0xf, false);
+}
+template <int... U>
+__device__ __forceinline__ void row16_bcast8(int c, int w, int (&cs)[8], int (&ws)[8],
+                                             std::integer_sequence<int, U...>) {
+    ((cs[U] = row16_bcast<U>(c)), ...);
+    ((ws[U] = row16_bcast<U>(w)), ...);
+}
 template <typename V>
 __device__ __forceinline__ int val_bits(V v) {
     if constexpr (sizeof(V) == 4) return __float_as_int(v);

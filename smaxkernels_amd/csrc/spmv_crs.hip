@@ -239,23 +239,28 @@ __global__ __launch_bounds__(kCrsThreads) void crs_spmm_rowvec(
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = 0;
     int64_t j = s;
-    if constexpr (GL == U && sizeof(V) <= 4) {
-        // 8 lanes per row (k = 32 with 4-byte X entries): lane l loads entry
-        // l of each group of 8 and a swizzle within the 8 lanes hands every
-        // lane all of them, instead of all 8 loading each; the next group's
-        // are requested before this group's X rows
+    if constexpr ((GL == U || GL == 2 * U) && sizeof(V) <= 4) {
+        // 8 or 16 lanes per row (k = 32 or 64 with 4-byte X entries): lane l
+        // loads entry l % 8 of each group of 8 and a swizzle within the 8
+        // lanes (a DPP row broadcast within the 16) hands every lane all of
+        // them, instead of every lane loading each; the next group's are
+        // requested before this group's X rows
         if (e - s >= U) {
-            int cm = col[j + l];
-            int wm = val_bits<V>(val[j + l]);
+            const int le = l % U;  // the entry of each group this lane loads
+            int cm = col[j + le];
+            int wm = val_bits<V>(val[j + le]);
             for (;;) {
                 const bool more = j + 2 * U <= e;
                 int cn = 0, wn = 0;
                 if (more) {
-                    cn = col[j + U + l];
-                    wn = val_bits<V>(val[j + U + l]);
+                    cn = col[j + U + le];
+                    wn = val_bits<V>(val[j + U + le]);
                 }
                 int cs[U], ws[U];
-                grp8_bcast8(cm, wm, cs, ws, std::make_integer_sequence<int, U>{});
+                if constexpr (GL == U)
+                    grp8_bcast8(cm, wm, cs, ws, std::make_integer_sequence<int, U>{});
+                else
+                    row16_bcast8(cm, wm, cs, ws, std::make_integer_sequence<int, U>{});
                 A xv[U][Q];
 #pragma unroll
                 for (int u = 0; u < U; ++u)

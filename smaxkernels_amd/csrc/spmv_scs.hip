@@ -392,27 +392,32 @@ __global__ __launch_bounds__(kScsThreads) void scs_spmm_vec(
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = 0;
     int j = 0;
-    if constexpr (GL == U && sizeof(V) <= 4) {
-        // K / Q = 8 lanes per slot (k = 32 with 4-byte X entries): every lane
-        // would load the same column index and value of each element, a
-        // third of the bytes the slot requests.  Instead lane l loads element
-        // l of each group of 8 and a swizzle within the 8 lanes hands every
-        // lane all of them; the next group's are requested before this
-        // group's X rows (lomax band N = 1M, k = 32: FP32 values 0.378 ->
-        // 0.371 ms, A^T 0.213 -> 0.200 ms; BF16 values 0.389 -> 0.372 ms).
+    if constexpr ((GL == U || GL == 2 * U) && sizeof(V) <= 4) {
+        // K / Q = 8 or 16 lanes per slot (k = 32 or 64 with 4-byte X
+        // entries): every lane would load the same column index and value of
+        // each element, a third of the bytes the slot requests.  Instead lane
+        // l loads element l % 8 of each group of 8 and a swizzle within the 8
+        // lanes (a DPP row broadcast within the 16) hands every lane all of
+        // them; the next group's are requested before this group's X rows
+        // (lomax band N = 1M, k = 32: FP32 values 0.378 -> 0.371 ms, A^T
+        // 0.213 -> 0.200 ms; BF16 values 0.389 -> 0.372 ms).
         if (len >= U) {
-            int cm = col[base + static_cast<int64_t>(l) * C];
-            int wm = val_bits<V>(val[base + static_cast<int64_t>(l) * C]);
+            const int le = l % U;  // the element of each group this lane loads
+            int cm = col[base + static_cast<int64_t>(le) * C];
+            int wm = val_bits<V>(val[base + static_cast<int64_t>(le) * C]);
             for (;;) {
                 const bool more = j + 2 * U <= len;
                 int cn = 0, wn = 0;
                 if (more) {
-                    const int64_t o = base + static_cast<int64_t>(j + U + l) * C;
+                    const int64_t o = base + static_cast<int64_t>(j + U + le) * C;
                     cn = col[o];
                     wn = val_bits<V>(val[o]);
                 }
                 int cs[U], ws[U];
-                grp8_bcast8(cm, wm, cs, ws, std::make_integer_sequence<int, U>{});
+                if constexpr (GL == U)
+                    grp8_bcast8(cm, wm, cs, ws, std::make_integer_sequence<int, U>{});
+                else
+                    row16_bcast8(cm, wm, cs, ws, std::make_integer_sequence<int, U>{});
                 A xv[U][Q];
 #pragma unroll
                 for (int u = 0; u < U; ++u)

@@ -104,7 +104,7 @@ def test_crs_spmv_long_and_empty_rows(ctx, vt):
 
 
 @pytest.mark.parametrize("vt", VTS)
-@pytest.mark.parametrize("k", [1, 4, 8, 32])
+@pytest.mark.parametrize("k", [1, 4, 8, 32, 64])
 @pytest.mark.parametrize("row_major", [True, False])
 @pytest.mark.parametrize("planned", [False, True])
 def test_crs_spmm_bit_exact(ctx, vt, k, row_major, planned):

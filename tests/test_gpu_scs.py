@@ -411,7 +411,7 @@ def test_scs_spmm_bit_exact(ctx, vt, k):
 
 @pytest.mark.parametrize("vt", [capi.F64, capi.F32, capi.BF16])
 @pytest.mark.parametrize("C_,sigma", [(64, 1), (64, 4096), (32, 1 << 20), (4, 16)])
-@pytest.mark.parametrize("k", [1, 3, 32, 70])
+@pytest.mark.parametrize("k", [1, 3, 32, 64, 70])
 def test_scs_apply_spmm_equals_crs_spmm(ctx, vt, C_, sigma, k):
     """Handle-level SpMM runs each output over the row's stored entries only:
     it equals the reference CRS SpMM bit for bit even with inf/NaN in X's
