@@ -337,28 +337,47 @@ __device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
         __builtin_amdgcn_mov_dpp(static_cast<int>(v), S * 0x55, 0xf, 0xf, false));
 }
 
-// bcrs_spmm_reg for dense 4x4 blocks with four-byte columns and 4 lanes per
-// block row (k = 8 in column pairs, Q = 2, or k = 16 in 16-byte runs, Q =
-// 4), so that a block row's lanes are one DPP quad: each lane loads a
-// quarter of the block (8 bytes of BF16 values, 16 of FP32) and quad
-// broadcasts give every lane the whole block, instead of every lane loading
-// all of it.  Blocks go in groups of 4 (the quarters take few registers),
+// Dword v of lane S of each aligned group of NL lanes (4: a DPP quad, 8: a
+// ds_swizzle group), handed to the group's lanes.
+template <int NL, int S>
+__device__ __forceinline__ uint32_t group_bcast(uint32_t v) {
+    if constexpr (NL == 4)
+        return quad_bcast<S>(v);
+    else
+        return static_cast<uint32_t>(grp8_bcast<S>(static_cast<int>(v)));
+}
+
+template <int NL, int M, int... S>
+__device__ __forceinline__ void group_bcast_all(const uint32_t (&mine)[M],
+                                                uint32_t (&words)[NL * M],
+                                                std::integer_sequence<int, S...>) {
+#pragma unroll
+    for (int d = 0; d < M; ++d) ((words[S * M + d] = group_bcast<NL, S>(mine[d])), ...);
+}
+
+// bcrs_spmm_reg for dense 4x4 blocks with four-byte columns and NL = 4 or 8
+// lanes per block row (k = 8 in column pairs, Q = 2; k = 16 or 32 in
+// 16-byte runs, Q = 4), so that a block row's lanes are one DPP quad or one
+// swizzle group of 8: each lane loads 1/NL of the block and broadcasts
+// within the group give every lane the whole block, instead of every lane
+// loading all of it.  Blocks go in groups of 4 (the quarters take few registers),
 // with the next group's column indices requested while this group's X rows
 // are in flight.  Per output the fma order is bcrs_spmm_reg's (block j, then
 // jx), so Y is bit-identical.
-template <int VT, int Q, bool CM, int G>
+template <int VT, int Q, int NL, bool CM, int G>
 __global__ __launch_bounds__(kBThreads) void bcrs_spmm_quad(
     int64_t n_block_rows, int64_t k, const int32_t *__restrict__ col,
     const int32_t *__restrict__ row_ptr, const typename vt<VT>::val_t *__restrict__ val,
     const float *__restrict__ X, float *__restrict__ Y) {
     using T = vt<VT>;
     using V = typename T::val_t;
-    constexpr int BH = 4, BW = 4, kQuad = 4;
+    constexpr int BH = 4, BW = 4;
     constexpr int kWords = BH * BW * static_cast<int>(sizeof(V)) / 4;  // dwords per block
-    constexpr int kMine = kWords / kQuad;                              // loaded per lane
+    constexpr int kMine = kWords / NL;                                 // loaded per lane
+    static_assert(kMine * NL == kWords, "a block splits evenly over the group");
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kBThreads + threadIdx.x;
-    const int64_t r = t / kQuad;  // whole quads leave together
-    const int l = static_cast<int>(threadIdx.x) % kQuad;
+    const int64_t r = t / NL;  // whole groups leave together
+    const int l = static_cast<int>(threadIdx.x) % NL;
     const int64_t v = static_cast<int64_t>(l) * Q;
     if (r >= n_block_rows) return;
     float acc[BH][Q];
@@ -392,13 +411,7 @@ __global__ __launch_bounds__(kBThreads) void bcrs_spmm_quad(
         for (int g = 0; g < G; ++g)
             if (j + g < je) {
                 uint32_t words[kWords];
-#pragma unroll
-                for (int d = 0; d < kMine; ++d) {
-                    words[0 * kMine + d] = quad_bcast<0>(mine[g][d]);
-                    words[1 * kMine + d] = quad_bcast<1>(mine[g][d]);
-                    words[2 * kMine + d] = quad_bcast<2>(mine[g][d]);
-                    words[3 * kMine + d] = quad_bcast<3>(mine[g][d]);
-                }
+                group_bcast_all<NL>(mine[g], words, std::make_integer_sequence<int, NL>{});
                 float w[BH * BW];
 #pragma unroll
                 for (int i = 0; i < BH * BW; ++i) {
@@ -465,20 +478,26 @@ static void launch_bcrs_spmm_reg(smax_hip_ctx ctx, int64_t n_block_rows, int64_t
         // each block per lane and DPP quad broadcasts, in groups of 4 blocks
         // (k = 8: BF16 values 0.152 -> 0.139 ms forward and through the
         // transpose, FP32 0.183 -> 0.160 ms; groups of 2: 0.150 / 0.171 ms;
-        // k = 16, FP32 values: 0.240 -> 0.223 ms)
-#define SMAX_BCRS_QUAD(QQ)                                                                 \
+        // k = 16, FP32 values: 0.240 -> 0.223 ms).  8 lanes (k = 32 in runs):
+        // an eighth per lane and swizzles within the 8 (BF16 values 0.331 ->
+        // 0.318 ms, FP32 0.368 -> 0.345 ms)
+#define SMAX_BCRS_QUAD(QQ, NN)                                                             \
     if (cm)                                                                                \
-        hipLaunchKernelGGL((bcrs_spmm_quad<VT, QQ, true, 4>), g, dim3(kBThreads), 0,       \
+        hipLaunchKernelGGL((bcrs_spmm_quad<VT, QQ, NN, true, 4>), g, dim3(kBThreads), 0,   \
                            ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv);         \
     else                                                                                   \
-        hipLaunchKernelGGL((bcrs_spmm_quad<VT, QQ, false, 4>), g, dim3(kBThreads), 0,      \
+        hipLaunchKernelGGL((bcrs_spmm_quad<VT, QQ, NN, false, 4>), g, dim3(kBThreads), 0,  \
                            ctx->stream, n_block_rows, k, col, row_ptr, v, xv, yv)
         if (pairs && lanes == 4) {
-            SMAX_BCRS_QUAD(2);
+            SMAX_BCRS_QUAD(2, 4);
             return;
         }
         if (runs && dense && lanes == 4) {
-            SMAX_BCRS_QUAD(4);
+            SMAX_BCRS_QUAD(4, 4);
+            return;
+        }
+        if (runs && dense && lanes == 8) {
+            SMAX_BCRS_QUAD(4, 8);
             return;
         }
 #undef SMAX_BCRS_QUAD

@@ -100,7 +100,8 @@ def test_block7_conversion_spmv_spmm(ctx, vt, bh, bw, hp, wp, cm):
     # shift 1: X and Y one entry past a 16-byte boundary (no 16-byte runs; the
     # same chains).  k = 8, 10 and misaligned 16 with 4-byte X entries and
     # dense blocks take column pairs per lane
-    for k, shift in ((1, 0), (6, 0), (8, 0), (8, 1), (10, 0), (16, 0), (16, 1), (64, 0)):
+    for k, shift in ((1, 0), (6, 0), (8, 0), (8, 1), (10, 0), (16, 0), (16, 1), (32, 0),
+                     (64, 0)):
         X = matgen.uniform_f64(4, i.n_block_cols * wp * k).astype(xt)
         n_y = i.n_block_rows * hp * k
         dX = ctx.upload(np.concatenate([X[:shift], X]))
@@ -118,7 +119,7 @@ def test_block7_conversion_spmv_spmm(ctx, vt, bh, bw, hp, wp, cm):
 
 @pytest.mark.parametrize("vt", [capi.F32, capi.BF16])
 @pytest.mark.parametrize("cm", [True, False])
-@pytest.mark.parametrize("k", [8, 16, 64])
+@pytest.mark.parametrize("k", [8, 16, 32, 64])
 def test_bcrs_spmm_ragged_block_rows(ctx, vt, cm, k):
     """Dense 4x4 blocks at k = 64 (one value per lane, broadcast across the
     block row's 16 lanes) and k = 8 and 16 (a quarter block per lane,
